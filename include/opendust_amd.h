@@ -1,0 +1,203 @@
+/*
+ * opendust_amd.h -- C ABI of the MI355X (gfx950) ion-flow engine.
+ *
+ * The engine replaces the per-timestep hot path of OpenDust (opendust.py, the
+ * OpenMM System/CustomIntegrator built in OpenDust.simulate and driven by
+ * integrator.step(1) at opendust.py:1059-1061) with hand-written HIP kernels.
+ * The Python layer (opendust_amd/opendust.py) keeps OpenDust's public classes
+ * and calls these functions through ctypes.
+ *
+ * Units inside the engine are the ones the reference hands to OpenMM
+ * (opendust.py:274-312): lengths in nm, time in ps, masses in amu, energies in
+ * kJ/mol, forces in kJ/mol/nm, velocities in nm/ps.  Dust charges and the
+ * per-step dust records are SI (C, N), as in OpenDust.dustParticles.
+ *
+ * Conventions: every function returns 0 on success and a non-zero status on
+ * failure; od_last_error() then returns a thread-local message.  Host buffers
+ * are owned by the caller and never retained.  One context = one HIP device +
+ * one HIP stream; a context is not thread-safe, distinct contexts are.
+ */
+#ifndef OPENDUST_AMD_H
+#define OPENDUST_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OD_ABI_VERSION 1
+#define OD_MAX_DUST 8
+#define OD_MAX_TABLE 512
+
+/* status codes */
+#define OD_OK 0
+#define OD_ERR_ARG 1
+#define OD_ERR_HIP 2
+#define OD_ERR_NONFINITE 3
+#define OD_ERR_STATE 4
+
+/* distribution types (OpenDust distributionType, opendust.py:300-305) */
+#define OD_DIST_MAXWELLIAN 0
+#define OD_DIST_FIELD_DRIVEN 1
+
+/* pair-force precision */
+#define OD_PREC_FP32C 0 /* FP32 pair arithmetic, compensated (Kahan) per-ion sums */
+#define OD_PREC_FP64 1  /* plain FP64 pair arithmetic */
+
+/* charge methods (opendust.py:1120, :1153; any other string keeps q) */
+#define OD_CHARGE_GIVEN 0
+#define OD_CHARGE_CONSISTENT 1
+#define OD_CHARGE_OML 2
+
+/* Piecewise-linear inverse CDF in the form the reference emits as an OpenMM
+ * expression (opendust.py:702-723, :764-785):
+ *   V(u) = sum_k (v0[k] + (u - xi0[k]) * slope[k]) * step(half[k] - |u - center[k]|)
+ * with step(t) = 1 for t >= 0.  The constants are the %.8f-rounded values the
+ * reference prints into the expression.  n_seg == 0 means "table absent". */
+typedef struct od_table {
+  int32_t n_seg;
+  int32_t pad_;
+  double v0[OD_MAX_TABLE];
+  double xi0[OD_MAX_TABLE];
+  double slope[OD_MAX_TABLE];
+  double center[OD_MAX_TABLE];
+  double half[OD_MAX_TABLE];
+} od_table;
+
+/* Everything the step needs; filled by the Python host layer. */
+typedef struct od_config {
+  int32_t abi_version;   /* = OD_ABI_VERSION */
+  int32_t n_ions;        /* N superions */
+  int32_t n_dust;        /* 0 .. OD_MAX_DUST */
+  int32_t distribution;  /* OD_DIST_* */
+  int32_t precision;     /* OD_PREC_* */
+  int32_t device;        /* HIP device ordinal */
+  int32_t use_bottom;    /* reference applies the bottom-face selects only if p_b != 0 (opendust.py:939) */
+  int32_t j_split;       /* pair kernel j-chunks; 0 = automatic */
+  uint64_t seed;         /* Philox4x32-10 key */
+
+  /* geometry, integration (OpenMM units) */
+  double R, H, dt;       /* nm, nm, ps */
+  double r_D_e, r_0;     /* nm */
+  double m_s;            /* superion mass, amu */
+  double v_T, v_fl;      /* nm/ps */
+
+  /* force prefactors (OpenMM units) */
+  double kqq;            /* kq_sq_s of the ion-ion law, kJ/mol*nm (opendust.py:472-479) */
+  double k_dust;         /* k*conv*q_s of the dust->ion law: U = q_p*k_dust/r (opendust.py:351-359) */
+  double trap_C;         /* prefactor of the trap polynomial, kJ/mol (opendust.py:552-557) */
+  double trap_alpha[15]; /* fitted coefficients alphaTrap[0..14] (opendust.py:547) */
+  double field_force;    /* E*q_s*conv, kJ/mol/nm (fieldDriven, opendust.py:580-585) */
+
+  /* Philox decision thresholds on a u32 draw, precomputed from p_t, p_s, p_b, dt*w_c */
+  uint64_t thr_top;      /* face = top    if u32 <  thr_top */
+  uint64_t thr_side;     /* face = side   if thr_top <= u32 < thr_side, else bottom */
+  uint64_t thr_coll;     /* collision on a DOF if u32 < thr_coll (fieldDriven) */
+
+  /* dust particles: nm for the ion side, SI for the dust-force assembly */
+  double dust_x[OD_MAX_DUST], dust_y[OD_MAX_DUST], dust_z[OD_MAX_DUST], dust_r[OD_MAX_DUST];
+  double dust_q0[OD_MAX_DUST];      /* initial charge, C */
+  int32_t dust_method[OD_MAX_DUST]; /* OD_CHARGE_* */
+  double si_dust_x[OD_MAX_DUST], si_dust_y[OD_MAX_DUST], si_dust_z[OD_MAX_DUST], si_dust_r[OD_MAX_DUST];
+
+  /* SI constants of the host-side per-step physics (opendust.py:1117-1384) */
+  double si_k;           /* 1/(4 pi eps_0) */
+  double si_e;           /* electron charge (negative) */
+  double si_q_s;         /* superion charge |e|*N_s */
+  double si_r_D_e;       /* m */
+  double si_dt;          /* s */
+  double si_kB, si_Te, si_Tn;
+  double si_E;           /* external field, V/m (fieldDriven) */
+  double gamma_e_pref;   /* n_inf*(T_e*k_B/pi/m_e/2)**0.5 (opendust.py:1136-1140) */
+  double oml_A;          /* n_inf*v_ti*u/4 */
+  double oml_B;          /* (1 + 1/(2u*2)) */
+  double oml_u2;         /* u**2 */
+  double oml_erf_u;      /* erf(u) */
+  double oml_tail;       /* 1/u/sqrt(pi)*exp(-u**2) */
+  double collect_scale;  /* (nm/ps -> m/s) * m_i * N_s / d_t (opendust.py:1344-1361) */
+
+  od_table top, bottom, side, vz;
+} od_config;
+
+/* Per-step, per-dust record (SI).  Field names follow OpenDust.dustParticles[p]. */
+typedef struct od_dust_record {
+  double q;                  /* dustParticles[p].q[i] */
+  double force_orbit[3];     /* forceIonsOrbit[i] */
+  double force_collect[3];   /* forceIonsCollect[i] */
+  double force_dust[3];      /* forceDust[i] */
+  double force_field[3];     /* forceExternalField[i] (fieldDriven) */
+  double orbit_sum[3];       /* raw sum over ions (reference's orbit_f_*_p / 3) */
+  double momentum_sum[3];    /* raw sum of v over absorbed ions, nm/ps (moment_f_*_p / 3) */
+  int64_t absorbed;          /* ions inside this dust this step (delta_q_p / 3) */
+} od_dust_record;
+
+typedef struct od_ctx od_ctx;
+
+/* Replaces mm.Context(system, integrator, platform, properties)
+ * (opendust.py:1039-1042): allocates device state for cfg->n_ions ions. */
+int od_create(const od_config* cfg, od_ctx** out);
+
+/* Replaces context.setPositions / setVelocities (opendust.py:1043-1044).
+ * x, v: [n][3] doubles, nm and nm/ps.  Recomputes the ion-ion force. */
+int od_set_ions(od_ctx* ctx, const double* x, const double* v, int64_t n);
+
+/* Replaces context.getState(getPositions, getVelocities) (opendust.py:1074-1078, :1103-1105). */
+int od_get_ions(od_ctx* ctx, double* x, double* v);
+
+/* Replaces context.setParameter("q<p>", ...) (opendust.py:1151, :1202): q[n_dust], C. */
+int od_set_dust_charges(od_ctx* ctx, const double* q);
+int od_get_dust_charges(od_ctx* ctx, double* q);
+
+/* Sets the loop index i of the next step (drives the "i > 0" charge rule and the
+ * Philox step counter).  od_create starts at 0. */
+int od_set_step(od_ctx* ctx, int64_t step);
+
+/* Replaces nsteps iterations of the loop body at opendust.py:1059-1384 without
+ * file output: integrator.step(1) plus the per-dust charge update and force
+ * assembly.  out: caller-owned [nsteps][n_dust] records (may be NULL).
+ * flags: OD_RUN_* below. */
+#define OD_RUN_GRAPH 1        /* replay the step as a captured hipGraph */
+#define OD_RUN_PROFILE_PAIR 2 /* time every ion-ion pair launch with HIP events */
+int od_run(od_ctx* ctx, int64_t nsteps, od_dust_record* out, int32_t flags);
+
+/* Accumulated HIP-event time of the main pair launches timed with
+ * OD_RUN_PROFILE_PAIR since the last reset, and their count. */
+int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
+
+/* Diagnostics of the last step (for parity tests): per-ion flag word
+ * (bit p < 16: inside dust p; bit 16: outside in r; bit 17: outside in z),
+ * per-ion reinjection face (0 none, 1 top, 2 side, 3 bottom), and the count. */
+int od_get_step_info(od_ctx* ctx, uint32_t* flags, uint8_t* face, int64_t* n_reinjected);
+
+/* Ion-ion force of the reference law on a given state, with the engine's
+ * kernels and precision (opendust.py:472-479): x [n][3] nm -> f [n][3] kJ/mol/nm. */
+int od_pair_forces(od_ctx* ctx, const double* x, double* f);
+
+/* Background potential of a uniform ion cylinder on the (rho, h) grid, Monte
+ * Carlo with n_mc Philox samples (opendust.py:499-523).  R, H, rho[n_r], h[n_h]
+ * in units of r_D_e (grid points (rho, 0, h)).
+ * out: [n_r][n_h] mean of exp(-d)/(d + 0.001). */
+int od_trap_potential_grid(int32_t device, double R, double H, int32_t n_r, const double* rho,
+                           int32_t n_h, const double* h, int64_t n_mc, uint64_t seed, double* out);
+
+/* Sample coordinates of the Monte Carlo above (for checking the sampler):
+ * k in [k0, k0 + n) -> xyz[n][3], units of r_D_e.  Pure host computation. */
+int od_trap_samples_host(double R, double H, uint64_t seed, int64_t k0, int64_t n, double* xyz);
+
+/* sizeof(od_config), sizeof(od_dust_record) as compiled (ABI check, no GPU use). */
+int od_abi_sizes(int64_t* config_size, int64_t* record_size);
+
+int od_destroy(od_ctx* ctx);
+
+/* Thread-local message of the last failure on this thread. */
+const char* od_last_error(void);
+
+/* Version string of the build (arch, ABI). */
+const char* od_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENDUST_AMD_H */

@@ -1,0 +1,1301 @@
+// od_engine.hip -- gfx950 kernels and the C ABI of the ion-flow engine.
+//
+// One step replaces one integrator.step(1) of the reference's OpenMM program
+// (opendust.py:839-1037) plus the host-side charge update and dust-force
+// assembly (opendust.py:1117-1384):
+//
+//   k_kick_drift   collisions, first half-kick, drift, absorption/outflow flags
+//   k_scan/k_list  ordered compaction of the flagged ions (ion-index order)
+//   k_pair32/64    ion-ion force at the drifted positions; flagged ions are
+//                  parked far away as sources, so this is the sum over kept ions
+//   k_kick2        + flagged-source correction, second half-kick, absorbed
+//                  counts and momenta, Philox reinjection, orbit sums
+//   k_rows         full force rows of the reinjected ions at their new place
+//   k_inflow       force of the reinjected ions on everybody else
+//   k_dust         FP64 reductions, charge update, per-step dust records
+//
+// The reference evaluates the ion-ion force twice per step (OpenMM recomputes
+// forces after the reinjection writes x).  Here the second evaluation costs
+// O(N * n_reinjected) instead of O(N^2) and involves no cancellation: the
+// kept-source sum is shared, only the flagged sources differ.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "od_device.h"
+
+using namespace od;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return set_err(OD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define CHKRET(expr)      \
+  do {                    \
+    int r_ = (expr);      \
+    if (r_ != OD_OK) return r_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device parameters (one copy per context, read-only in kernels)
+// ---------------------------------------------------------------------------
+struct DevParams {
+  ExtParams ext;
+  int n, n_pad, n_dust, distribution, use_bottom;
+  uint64_t seed;
+  double R, H, half_H, dt, half_dt, m_s, v_T, v_fl, lam;
+  double pos_scale;   // nm -> kernel units (FP32: L/lambda, FP64: 1/lambda)
+  double pair_scale;  // kernel accumulator -> kJ/mol/nm
+  float a32, linv32;
+  double rho0;
+  uint64_t thr_top, thr_side, thr_coll;
+  int dust_method[OD_MAX_DUST];
+  double si_k, si_e, si_q_s, si_lam, si_dt, si_kB, si_Te, si_Tn, si_E;
+  double gamma_e_pref, oml_A, oml_B, oml_u2, oml_erf_u, oml_tail, collect_scale;
+  double si_dx[OD_MAX_DUST], si_dy[OD_MAX_DUST], si_dz[OD_MAX_DUST], si_dr[OD_MAX_DUST];
+  od_table top, bottom, side, vz;
+};
+
+template <class T>
+struct Pos;
+template <>
+struct Pos<float> {
+  using V = float4;
+  __device__ static V make(double x, double y, double z, double s) {
+    return make_float4((float)(x * s), (float)(y * s), (float)(z * s), 0.0f);
+  }
+  __device__ static V park() { return make_float4(PARK32, PARK32, PARK32, 0.0f); }
+};
+template <>
+struct Pos<double> {
+  using V = double4;
+  __device__ static V make(double x, double y, double z, double s) { return make_double4(x * s, y * s, z * s, 0.0); }
+  __device__ static V park() { return make_double4(PARK64, PARK64, PARK64, 0.0); }
+};
+
+// Accumulator of one force component: FP32 with tile-wise Kahan, or FP64.
+template <class T>
+struct Acc;
+template <>
+struct Acc<float> {
+  float a = 0.f, s = 0.f, c = 0.f;
+  __device__ void flush() {
+    kahan_add(s, c, a);
+    a = 0.f;
+  }
+  __device__ double total() const { return (double)s - (double)c; }
+};
+template <>
+struct Acc<double> {
+  double a = 0.0;
+  __device__ void flush() {}
+  __device__ double total() const { return a; }
+};
+
+template <class T>
+__device__ __forceinline__ void pair_acc(const DevParams& P, const typename Pos<T>::V& i, const typename Pos<T>::V& j,
+                                         Acc<T>& ax, Acc<T>& ay, Acc<T>& az);
+template <>
+__device__ __forceinline__ void pair_acc<float>(const DevParams& P, const float4& i, const float4& j, Acc<float>& ax,
+                                                Acc<float>& ay, Acc<float>& az) {
+  pair32(i.x, i.y, i.z, j.x, j.y, j.z, P.a32, P.linv32, ax.a, ay.a, az.a);
+}
+template <>
+__device__ __forceinline__ void pair_acc<double>(const DevParams& P, const double4& i, const double4& j,
+                                                 Acc<double>& ax, Acc<double>& ay, Acc<double>& az) {
+  pair64(i.x, i.y, i.z, j.x, j.y, j.z, P.rho0, ax.a, ay.a, az.a);
+}
+
+// deterministic block sum of a double over a 256-thread block (value valid on all threads)
+__device__ double block_sum256(double v, double* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  const double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// positions -> kernel coordinates
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_load_pos(const double* __restrict__ X, int n, int n_pad, double s,
+                                                  typename Pos<T>::V* __restrict__ pi,
+                                                  typename Pos<T>::V* __restrict__ pj) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_pad) return;
+  typename Pos<T>::V p = (i < n) ? Pos<T>::make(X[i], X[n_pad + i], X[2 * n_pad + i], s) : Pos<T>::park();
+  pi[i] = p;
+  pj[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// all-pairs ion-ion force, FP32 with tile-compensated sums.
+// Block = 256 threads, IB i-ions per thread (stride 256).  j runs over one
+// chunk of the padded source array; source positions are wave-uniform, so they
+// arrive through scalar loads and feed the VALU as SGPR operands.  Blocks are
+// ordered chunk-fastest, so consecutive blocks (which the dispatcher spreads
+// over the 8 XCDs) stream different chunks and each XCD's L2 holds one chunk.
+// out: [n_chunks][3][n_pad] partial sums already scaled to kJ/mol/nm.
+// ---------------------------------------------------------------------------
+constexpr int TJ = 64;  // Kahan flush period (sources)
+
+template <int IB>
+__global__ __launch_bounds__(256) void k_pair32(const float4* __restrict__ pi, const float4* __restrict__ pj,
+                                                int n_pad, int n_chunks, int chunk_len, float a, float linv,
+                                                double scale, double* __restrict__ out) {
+  const int chunk = blockIdx.x % n_chunks;
+  const int iblk = blockIdx.x / n_chunks;
+  const int ibase = iblk * (256 * IB) + threadIdx.x;
+  float xi[IB], yi[IB], zi[IB];
+  float ax[IB], ay[IB], az[IB], sx[IB], sy[IB], sz[IB], cx[IB], cy[IB], cz[IB];
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    const float4 p = pi[ibase + k * 256];
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    sx[k] = sy[k] = sz[k] = 0.f;
+    cx[k] = cy[k] = cz[k] = 0.f;
+  }
+  const int j0 = chunk * chunk_len;
+  const int j1 = min(j0 + chunk_len, n_pad);
+  for (int jt = j0; jt < j1; jt += TJ) {
+#pragma unroll
+    for (int k = 0; k < IB; ++k) ax[k] = ay[k] = az[k] = 0.f;
+#pragma unroll 8
+    for (int jj = 0; jj < TJ; ++jj) {
+      const float4 p = pj[jt + jj];
+#pragma unroll
+      for (int k = 0; k < IB; ++k) pair32(xi[k], yi[k], zi[k], p.x, p.y, p.z, a, linv, ax[k], ay[k], az[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < IB; ++k) {
+      kahan_add(sx[k], cx[k], ax[k]);
+      kahan_add(sy[k], cy[k], ay[k]);
+      kahan_add(sz[k], cz[k], az[k]);
+    }
+  }
+  double* o = out + (size_t)chunk * 3 * n_pad;
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    const int i = ibase + k * 256;
+    o[i] = ((double)sx[k] - (double)cx[k]) * scale;
+    o[n_pad + i] = ((double)sy[k] - (double)cy[k]) * scale;
+    o[2 * n_pad + i] = ((double)sz[k] - (double)cz[k]) * scale;
+  }
+}
+
+// all-pairs ion-ion force, plain FP64 (lambda units)
+template <int IB>
+__global__ __launch_bounds__(256) void k_pair64(const double4* __restrict__ pi, const double4* __restrict__ pj,
+                                                int n_pad, int n_chunks, int chunk_len, double rho0, double scale,
+                                                double* __restrict__ out) {
+  const int chunk = blockIdx.x % n_chunks;
+  const int iblk = blockIdx.x / n_chunks;
+  const int ibase = iblk * (256 * IB) + threadIdx.x;
+  double xi[IB], yi[IB], zi[IB], ax[IB], ay[IB], az[IB];
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    const double4 p = pi[ibase + k * 256];
+    xi[k] = p.x;
+    yi[k] = p.y;
+    zi[k] = p.z;
+    ax[k] = ay[k] = az[k] = 0.0;
+  }
+  const int j0 = chunk * chunk_len;
+  const int j1 = min(j0 + chunk_len, n_pad);
+#pragma unroll 4
+  for (int j = j0; j < j1; ++j) {
+    const double4 p = pj[j];
+#pragma unroll
+    for (int k = 0; k < IB; ++k) pair64(xi[k], yi[k], zi[k], p.x, p.y, p.z, rho0, ax[k], ay[k], az[k]);
+  }
+  double* o = out + (size_t)chunk * 3 * n_pad;
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    const int i = ibase + k * 256;
+    o[i] = ax[k] * scale;
+    o[n_pad + i] = ay[k] * scale;
+    o[2 * n_pad + i] = az[k] * scale;
+  }
+}
+
+// F = sum over chunks (fixed order)
+__global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__ part, int n_chunks, int n3,
+                                                      double* __restrict__ F) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n3) return;
+  double s = part[i];
+  for (int c = 1; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
+  F[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// step kernels
+// ---------------------------------------------------------------------------
+constexpr uint32_t FLAG_OUT_R = 1u << 16;
+constexpr uint32_t FLAG_OUT_Z = 1u << 17;
+
+// Collisions (fieldDriven, opendust.py:869-873), first half-kick, drift, flags
+// (opendust.py:864-910).  pj gets the drifted position, or the parking spot
+// for flagged ions, which are excluded as sources of the main pair kernel.
+template <class T>
+__global__ __launch_bounds__(256) void k_kick_drift(const DevParams* __restrict__ P, double* __restrict__ X,
+                                                    double* __restrict__ V, const double* __restrict__ F,
+                                                    const double* __restrict__ q, const uint32_t* __restrict__ st,
+                                                    typename Pos<T>::V* __restrict__ pi,
+                                                    typename Pos<T>::V* __restrict__ pj, uint32_t* __restrict__ flags,
+                                                    int* __restrict__ block_cnt, int* __restrict__ err) {
+  const int n = P->n, np = P->n_pad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int flagged = 0;
+  if (i < n) {
+    double x = X[i], y = X[np + i], z = X[2 * np + i];
+    double vx = V[i], vy = V[np + i], vz = V[2 * np + i];
+    const uint32_t step = st[0];
+    if (P->distribution == OD_DIST_FIELD_DRIVEN) {
+      const u32x4 r = philox4x32_10((uint32_t)i, step, 0u, S_COLL, P->seed);
+      const bool c0 = r.v[0] < P->thr_coll, c1 = r.v[1] < P->thr_coll, c2 = r.v[2] < P->thr_coll;
+      if (c0 || c1 || c2) {
+        double g0, g1, g2, gd;
+        box_muller(philox4x32_10((uint32_t)i, step, 0u, S_COLL | (1u << 8), P->seed), g0, g1);
+        box_muller(philox4x32_10((uint32_t)i, step, 0u, S_COLL | (2u << 8), P->seed), g2, gd);
+        if (c0) vx = P->v_T * g0;
+        if (c1) vy = P->v_T * g1;
+        if (c2) vz = P->v_T * g2;
+      }
+    }
+    double fx, fy, fz;
+    ext_force(P->ext, q, x, y, z, fx, fy, fz);
+    fx += F[i];
+    fy += F[np + i];
+    fz += F[2 * np + i];
+    vx = vx + P->half_dt * fx / P->m_s;
+    vy = vy + P->half_dt * fy / P->m_s;
+    vz = vz + P->half_dt * fz / P->m_s;
+    x = x + P->dt * vx;
+    y = y + P->dt * vy;
+    z = z + P->dt * vz;
+    uint32_t fl = 0;
+    for (int p = 0; p < P->n_dust; ++p) {
+      const double dx = x - P->ext.dx[p], dy = y - P->ext.dy[p], dz = z - P->ext.dz[p];
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      if (-d + P->ext.dr[p] >= 0.0) fl |= 1u << p;
+    }
+    if (-P->R + sqrt(x * x + y * y) >= 0.0) fl |= FLAG_OUT_R;
+    if (-P->half_H + fabs(z) >= 0.0) fl |= FLAG_OUT_Z;
+    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(vx) && isfinite(vy) && isfinite(vz)))
+      atomicOr(err, 1);
+    X[i] = x;
+    X[np + i] = y;
+    X[2 * np + i] = z;
+    V[i] = vx;
+    V[np + i] = vy;
+    V[2 * np + i] = vz;
+    flags[i] = fl;
+    const typename Pos<T>::V p = Pos<T>::make(x, y, z, P->pos_scale);
+    pi[i] = p;
+    pj[i] = fl ? Pos<T>::park() : p;
+    flagged = fl != 0;
+  }
+  const int cnt = __syncthreads_count(flagged);
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = cnt;
+}
+
+// exclusive scan of the per-block flagged counts (one block)
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ block_cnt, int nb, int* __restrict__ block_off,
+                                               int* __restrict__ n_flag) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+  int local = 0;
+  for (int b = b0; b < b1; ++b) local += block_cnt[b];
+  s[t] = local;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - local;  // exclusive prefix of this thread's range
+  for (int b = b0; b < b1; ++b) {
+    block_off[b] = run;
+    run += block_cnt[b];
+  }
+  if (t == 1023) n_flag[0] = s[1023];
+}
+
+// ordered list of flagged ions: slot = offset of the block + rank inside it
+template <class T>
+__global__ __launch_bounds__(256) void k_list(const uint32_t* __restrict__ flags, const int* __restrict__ block_cnt,
+                                              const int* __restrict__ block_off, int n,
+                                              const typename Pos<T>::V* __restrict__ pi, int* __restrict__ slot_of,
+                                              int* __restrict__ list_ion, typename Pos<T>::V* __restrict__ list_old) {
+  if (block_cnt[blockIdx.x] == 0) return;
+  __shared__ int wcnt[4];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool f = (i < n) && flags[i] != 0u;
+  const unsigned long long m = __ballot(f);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r_in_wave = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wcnt[wave] = __popcll(m);
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wcnt[w];
+  if (f) {
+    const int s = block_off[blockIdx.x] + before + r_in_wave;
+    slot_of[i] = s;
+    list_ion[s] = i;
+    list_old[s] = pi[i];
+  }
+}
+
+// Reinjection of one flagged ion (opendust.py:923-975), FP64.
+// face: 1 top, 2 side, 3 bottom.
+__device__ int reinject(const DevParams& P, uint32_t ion, uint32_t step, double& x, double& y, double& z, double& vx,
+                        double& vy, double& vz) {
+  const double two_pi = 2.0 * 3.141592653589793;
+  const uint32_t uf = philox4x32_10(ion, step, 0u, S_FACE, P.seed).v[0];
+  int face;
+  if ((uint64_t)uf < P.thr_top)
+    face = 1;
+  else if ((uint64_t)uf < P.thr_side)
+    face = 2;
+  else
+    face = 3;
+  if (face == 1 || face == 3) {
+    const bool top = face == 1;
+    double g1, g2;
+    box_muller(philox4x32_10(ion, step, 0u, top ? S_TOPV : S_BOTV, P.seed), g1, g2);
+    const u32x4 rz = philox4x32_10(ion, step, 0u, (top ? S_TOPV : S_BOTV) | (1u << 8), P.seed);
+    const double uz = u53(rz.v[0], rz.v[1]);
+    vx = P.v_T * g1;
+    vy = P.v_T * g2;
+    vz = table_eval(top ? P.top : P.bottom, uz);
+    const u32x4 rx = philox4x32_10(ion, step, 0u, top ? S_TOPX : S_BOTX, P.seed);
+    const double ux = u53(rx.v[0], rx.v[1]), uy = u53(rx.v[2], rx.v[3]);
+    const double rr = P.R * sqrt(ux);
+    x = rr * cos(two_pi * uy);
+    y = rr * sin(two_pi * uy);
+    z = top ? -P.half_H : P.half_H;
+  } else {
+    const u32x4 rx = philox4x32_10(ion, step, 0u, S_SIDEX, P.seed);
+    const double uy = u53(rx.v[0], rx.v[1]), uz = u53(rx.v[2], rx.v[3]);
+    x = P.R * cos(two_pi * uy);
+    y = P.R * sin(two_pi * uy);
+    z = uz * P.H - P.half_H;
+    double gy, gz;
+    box_muller(philox4x32_10(ion, step, 0u, S_SIDEV, P.seed), gy, gz);
+    const u32x4 ru = philox4x32_10(ion, step, 0u, S_SIDEV | (1u << 8), P.seed);
+    const double us = u53(ru.v[0], ru.v[1]);
+    const double vs = table_eval(P.side, us);
+    vx = -vs * (x / P.R) + P.v_T * gy * (y / P.R);
+    vy = -vs * (y / P.R) - P.v_T * gy * (x / P.R);
+    if (P.distribution == OD_DIST_FIELD_DRIVEN)
+      vz = table_eval(P.vz, us);
+    else
+      vz = gz * P.v_T + P.v_fl;
+  }
+  return face;
+}
+
+// Second half-kick with the flagged-source correction, absorbed counts and
+// momenta (opendust.py:911-921), reinjection, orbit sums at the final
+// positions (opendust.py:977-1037).  partials: [nb][OD_MAX_DUST][8].
+template <class T>
+__global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, double* __restrict__ X,
+                                               double* __restrict__ V, const double* __restrict__ F,
+                                               const double* __restrict__ q, const uint32_t* __restrict__ st,
+                                               const typename Pos<T>::V* __restrict__ pi,
+                                               typename Pos<T>::V* __restrict__ pj,
+                                               const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
+                                               const int* __restrict__ slot_of,
+                                               const typename Pos<T>::V* __restrict__ list_old,
+                                               typename Pos<T>::V* __restrict__ list_new, uint8_t* __restrict__ face_out,
+                                               double* __restrict__ partials) {
+  __shared__ double lds[4];
+  const int n = P->n, np = P->n_pad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < n;
+  const typename Pos<T>::V own = valid ? pi[i] : Pos<T>::park();
+  Acc<T> ax, ay, az;
+  const int nf = n_flag[0];
+  for (int s0 = 0; s0 < nf; s0 += TJ) {
+    const int s1 = min(s0 + TJ, nf);
+    for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_old[s], ax, ay, az);
+    ax.flush();
+    ay.flush();
+    az.flush();
+  }
+  uint32_t fl = 0;
+  double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0, px = 0, py = 0, pz = 0;
+  if (valid) {
+    fl = flags[i];
+    x = X[i];
+    y = X[np + i];
+    z = X[2 * np + i];
+    vx = V[i];
+    vy = V[np + i];
+    vz = V[2 * np + i];
+    double fx, fy, fz;
+    ext_force(P->ext, q, x, y, z, fx, fy, fz);
+    fx += F[i] + ax.total() * P->pair_scale;
+    fy += F[np + i] + ay.total() * P->pair_scale;
+    fz += F[2 * np + i] + az.total() * P->pair_scale;
+    vx = vx + P->half_dt * fx / P->m_s;
+    vy = vy + P->half_dt * fy / P->m_s;
+    vz = vz + P->half_dt * fz / P->m_s;
+    px = vx;
+    py = vy;
+    pz = vz;
+    int face = 0;
+    if (fl) {
+      face = reinject(*P, (uint32_t)i, st[0], x, y, z, vx, vy, vz);
+      X[i] = x;
+      X[np + i] = y;
+      X[2 * np + i] = z;
+      const typename Pos<T>::V p = Pos<T>::make(x, y, z, P->pos_scale);
+      pj[i] = p;
+      list_new[slot_of[i]] = p;
+    }
+    V[i] = vx;
+    V[np + i] = vy;
+    V[2 * np + i] = vz;
+    face_out[i] = (uint8_t)face;
+  }
+  for (int p = 0; p < P->n_dust; ++p) {
+    const double in = (valid && ((fl >> p) & 1u)) ? 1.0 : 0.0;
+    double ox = 0, oy = 0, oz = 0;
+    if (valid) {
+      const double xp = P->ext.dx[p], yp = P->ext.dy[p], zp = P->ext.dz[p], lam = P->lam;
+      const double dx = x - xp, dy = y - yp, dz = z - zp;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const double e = exp(-r / lam);
+      const double r3 = r * r * r, r2 = r * r, l2 = lam * lam;
+      ox = e * (xp - x) / r3 * l2 + e * (xp - x) / r2 * lam;
+      oy = e * (yp - y) / r3 * l2 + e * (yp - y) / r2 * lam;
+      oz = e * (zp - z) / r3 * l2 + e * (zp - z) / r2 * lam;
+    }
+    const double v[7] = {in, in * px, in * py, in * pz, ox, oy, oz};
+    double* dst = partials + ((size_t)blockIdx.x * OD_MAX_DUST + p) * 8;
+    for (int k = 0; k < 7; ++k) {
+      const double s = block_sum256(v[k], lds);
+      if (threadIdx.x == 0) dst[k] = s;
+    }
+  }
+}
+
+// Full force rows of the reinjected ions at the post-reinjection state.
+template <class T>
+__global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ P,
+                                              const typename Pos<T>::V* __restrict__ pj, const int* __restrict__ n_flag,
+                                              const typename Pos<T>::V* __restrict__ list_new,
+                                              double* __restrict__ rowF) {
+  __shared__ double lds[4];
+  const int nf = n_flag[0];
+  const int np = P->n_pad;
+  for (int s = blockIdx.x; s < nf; s += gridDim.x) {
+    const typename Pos<T>::V own = list_new[s];
+    Acc<T> ax, ay, az;
+    for (int j0 = threadIdx.x; j0 < np; j0 += 256 * TJ) {
+      const int j1 = min(j0 + 256 * TJ, np);
+      for (int j = j0; j < j1; j += 256) pair_acc<T>(*P, own, pj[j], ax, ay, az);
+      ax.flush();
+      ay.flush();
+      az.flush();
+    }
+    const double fx = block_sum256(ax.total(), lds);
+    const double fy = block_sum256(ay.total(), lds);
+    const double fz = block_sum256(az.total(), lds);
+    if (threadIdx.x == 0) {
+      rowF[3 * s + 0] = fx * P->pair_scale;
+      rowF[3 * s + 1] = fy * P->pair_scale;
+      rowF[3 * s + 2] = fz * P->pair_scale;
+    }
+  }
+}
+
+// Force of the reinjected ions on the kept ions; reinjected ions take their rows.
+template <class T>
+__global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ P, double* __restrict__ F,
+                                                const typename Pos<T>::V* __restrict__ pj,
+                                                const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
+                                                const typename Pos<T>::V* __restrict__ list_new,
+                                                const int* __restrict__ slot_of, const double* __restrict__ rowF) {
+  const int n = P->n, np = P->n_pad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < n;
+  const typename Pos<T>::V own = valid ? pj[i] : Pos<T>::park();
+  Acc<T> ax, ay, az;
+  const int nf = n_flag[0];
+  for (int s0 = 0; s0 < nf; s0 += TJ) {
+    const int s1 = min(s0 + TJ, nf);
+    for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_new[s], ax, ay, az);
+    ax.flush();
+    ay.flush();
+    az.flush();
+  }
+  if (!valid) return;
+  if (flags[i]) {
+    const int s = slot_of[i];
+    F[i] = rowF[3 * s + 0];
+    F[np + i] = rowF[3 * s + 1];
+    F[2 * np + i] = rowF[3 * s + 2];
+  } else {
+    F[i] += ax.total() * P->pair_scale;
+    F[np + i] += ay.total() * P->pair_scale;
+    F[2 * np + i] += az.total() * P->pair_scale;
+  }
+}
+
+// Dust records: FP64 block-partial reduction (fixed order), charge update
+// (opendust.py:1117-1202), dust forces (opendust.py:1204-1384).
+__global__ __launch_bounds__(256) void k_dust(const DevParams* __restrict__ P, const double* __restrict__ partials,
+                                              int nb, double* __restrict__ q, uint32_t* __restrict__ st,
+                                              od_dust_record* __restrict__ rec) {
+  __shared__ double lds[4];
+  __shared__ double tot[OD_MAX_DUST][8];
+  const int nd = P->n_dust;
+  for (int p = 0; p < nd; ++p) {
+    for (int k = 0; k < 7; ++k) {
+      double s = 0.0;
+      for (int b = threadIdx.x; b < nb; b += 256) s += partials[((size_t)b * OD_MAX_DUST + p) * 8 + k];
+      s = block_sum256(s, lds);
+      if (threadIdx.x == 0) tot[p][k] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const uint32_t i = st[0];
+  const uint32_t ri = i - st[1];
+  const double pi_ = 3.141592653589793;
+  double qn[OD_MAX_DUST];
+  for (int p = 0; p < nd; ++p) {
+    const double qp = q[p];
+    qn[p] = qp;
+    const int m = P->dust_method[p];
+    if (i > 0 && (m == OD_CHARGE_CONSISTENT || m == OD_CHARGE_OML)) {
+      const double rp = P->si_dr[p];
+      const double phi = P->si_k * qp / rp;
+      const double gamma_e = P->gamma_e_pref * exp(-P->si_e * phi / P->si_Te / P->si_kB);
+      const double S = rp * rp * 4.0 * pi_;
+      double dq;
+      if (m == OD_CHARGE_CONSISTENT) {
+        dq = tot[p][0] * P->si_q_s + gamma_e * P->si_dt * S * P->si_e;
+      } else {
+        const double chi = P->si_e * phi / P->si_Tn / P->si_kB;
+        const double gamma_i = P->oml_A * ((P->oml_B + chi / P->oml_u2) * P->oml_erf_u + P->oml_tail);
+        dq = gamma_i * P->si_dt * S * fabs(P->si_e) + gamma_e * P->si_dt * S * P->si_e;
+      }
+      qn[p] = qp + dq;
+    }
+  }
+  const double lam = P->si_lam, k = P->si_k, qs = P->si_q_s;
+  const double* al = P->ext.alpha;
+  for (int p = 0; p < nd; ++p) {
+    od_dust_record r;
+    r.q = qn[p];
+    const double xp = P->si_dx[p], yp = P->si_dy[p], zp = P->si_dz[p];
+    double fdx = 0, fdy = 0, fdz = 0;
+    for (int j = 0; j < nd; ++j) {
+      if (j == p) continue;
+      const double dx = xp - P->si_dx[j], dy = yp - P->si_dy[j], dz = zp - P->si_dz[j];
+      const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+      const double r3 = rr * rr * rr;
+      fdx += k * qn[p] * qn[j] * dx / r3;
+      fdy += k * qn[p] * qn[j] * dy / r3;
+      fdz += k * qn[p] * qn[j] * dz / r3;
+    }
+    r.force_dust[0] = fdx;
+    r.force_dust[1] = fdy;
+    r.force_dust[2] = fdz;
+    // in-domain (orbit) part
+    const double cq = qn[p] * k * qs;
+    const double l2 = lam * lam;
+    double fo[3];
+    for (int c = 0; c < 3; ++c) fo[c] = tot[p][4 + c] * cq / l2;
+    // out-of-domain part: analytic -grad of the trap polynomial at the dust
+    const double rr = sqrt(xp * xp + yp * yp);
+    const double z = zp;
+    const double r2 = rr * rr, z2 = z * z, l4 = l2 * l2, l6 = l4 * l2, l8 = l4 * l4;
+    const double r3 = r2 * rr, r5 = r3 * r2, r7 = r5 * r2, r4 = r2 * r2, r6 = r4 * r2;
+    const double z3 = z2 * z, z4 = z2 * z2, z5 = z4 * z, z6 = z4 * z2, z7 = z6 * z;
+    double fR = (-2 * al[1] * rr) / l2;
+    fR += (-2 * al[3] * rr * z2 - 4 * al[4] * r3) / l4;
+    fR += (-4 * al[6] * r3 * z2 - 2 * al[7] * rr * z4 - 6 * al[8] * r5) / l6;
+    fR += (-4 * al[10] * r3 * z4 - 6 * al[11] * r5 * z2 - 2 * al[12] * rr * z6 - 8 * al[13] * r7) / l8;
+    fR *= k * qn[p] * qs / lam;
+    double fH = (-2 * al[2] * z) / l2;
+    fH += (-2 * al[3] * r2 * z - 4 * al[5] * z3) / l4;
+    fH += (-2 * al[6] * r4 * z - 4 * al[7] * r2 * z3 - 6 * al[9] * z5) / l6;
+    fH += (-4 * al[10] * r4 * z3 - 2 * al[11] * r6 * z - 6 * al[12] * r2 * z5 - 8 * al[14] * z7) / l8;
+    fH *= k * qn[p] * qs / lam;
+    double ofx = 0, ofy = 0;
+    if (rr != 0.0) {
+      ofx = fR * xp / rr;
+      ofy = fR * yp / rr;
+    }
+    r.force_orbit[0] = fo[0] + ofx;
+    r.force_orbit[1] = fo[1] + ofy;
+    r.force_orbit[2] = fo[2] + fH;
+    for (int c = 0; c < 3; ++c) {
+      r.force_collect[c] = tot[p][1 + c] * P->collect_scale;
+      r.orbit_sum[c] = tot[p][4 + c];
+      r.momentum_sum[c] = tot[p][1 + c];
+    }
+    r.force_field[0] = 0.0;
+    r.force_field[1] = 0.0;
+    r.force_field[2] = (P->distribution == OD_DIST_FIELD_DRIVEN) ? qn[p] * P->si_E : 0.0;
+    r.absorbed = (int64_t)tot[p][0];
+    rec[(size_t)ri * nd + p] = r;
+  }
+  for (int p = 0; p < nd; ++p) q[p] = qn[p];
+  st[0] = i + 1;
+}
+
+// ---------------------------------------------------------------------------
+// trap background potential (Monte Carlo, opendust.py:499-523)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void trap_sample(double R, double H, uint64_t seed, int64_t k, double& x, double& y,
+                                            double& z) {
+  const u32x4 a = philox4x32_10((uint32_t)k, (uint32_t)((uint64_t)k >> 32), 0u, S_TRAP, seed);
+  const u32x4 b = philox4x32_10((uint32_t)k, (uint32_t)((uint64_t)k >> 32), 0u, S_TRAP | (1u << 8), seed);
+  const double r = sqrt(u53(a.v[0], a.v[1]));
+  const double theta = u53(a.v[2], a.v[3]);
+  x = R * r * cos(2.0 * 3.141592653589793 * theta);
+  y = R * r * sin(2.0 * 3.141592653589793 * theta);
+  z = -H / 2.0 + (H / 2.0 - (-H / 2.0)) * u53(b.v[0], b.v[1]);
+}
+
+__global__ __launch_bounds__(256) void k_trap_samples(double R, double H, uint64_t seed, int64_t n,
+                                                      double* __restrict__ xs, double* __restrict__ ys,
+                                                      double* __restrict__ zs) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  double x, y, z;
+  trap_sample(R, H, seed, k, x, y, z);
+  xs[k] = x;
+  ys[k] = y;
+  zs[k] = z;
+}
+
+__global__ __launch_bounds__(256) void k_trap_grid(const double* __restrict__ rho, const double* __restrict__ hh,
+                                                   int n_h, const double* __restrict__ xs,
+                                                   const double* __restrict__ ys, const double* __restrict__ zs,
+                                                   int64_t n, int64_t chunk, int n_chunks, double* __restrict__ part) {
+  __shared__ double lds[4];
+  const int gp = blockIdx.x / n_chunks, c = blockIdx.x % n_chunks;
+  const double gR = rho[gp / n_h], gH = hh[gp % n_h];
+  const int64_t k0 = (int64_t)c * chunk, k1 = min(n, k0 + chunk);
+  double s = 0.0;
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const double dx = gR - xs[k], dz = gH - zs[k], y = ys[k];
+    const double d = sqrt(dx * dx + y * y + dz * dz);
+    s += exp(-d) / (d + 0.001);
+  }
+  s = block_sum256(s, lds);
+  if (threadIdx.x == 0) part[(size_t)gp * n_chunks + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_trap_finish(const double* __restrict__ part, int n_gp, int n_chunks,
+                                                     double n_mc, double* __restrict__ out) {
+  const int gp = blockIdx.x * 256 + threadIdx.x;
+  if (gp >= n_gp) return;
+  double s = 0.0;
+  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)gp * n_chunks + c];
+  out[gp] = s / n_mc;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct od_ctx {
+  od_config cfg;
+  int dev = 0;
+  int prec = OD_PREC_FP32C;
+  hipStream_t stream = nullptr;
+  int n = 0, n_pad = 0, nb = 0, n_chunks = 1, chunk_len = 0, ib = 2, rows_grid = 1;
+  size_t pos_bytes = 16;
+  DevParams* dP = nullptr;
+  double *X = nullptr, *V = nullptr, *F = nullptr, *Fpart = nullptr, *q = nullptr, *partials = nullptr,
+         *rowF = nullptr;
+  void *pi = nullptr, *pj = nullptr, *list_old = nullptr, *list_new = nullptr;
+  uint32_t *flags = nullptr, *st = nullptr;
+  uint8_t* face = nullptr;
+  int *block_cnt = nullptr, *block_off = nullptr, *n_flag = nullptr, *slot_of = nullptr, *list_ion = nullptr,
+      *err = nullptr;
+  od_dust_record* rec = nullptr;
+  int rec_cap = 1024;
+  int64_t host_step = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_steps = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pair_ms = 0.0;
+  int64_t pair_launches = 0;
+};
+
+static int fill_params(const od_config* c, int n_pad, DevParams* P) {
+  std::memset(P, 0, sizeof(*P));
+  const double L = 1.4426950408889634;
+  P->n = c->n_ions;
+  P->n_pad = n_pad;
+  P->n_dust = c->n_dust;
+  P->distribution = c->distribution;
+  P->use_bottom = c->use_bottom;
+  P->seed = c->seed;
+  P->R = c->R;
+  P->H = c->H;
+  P->half_H = c->H / 2.0;
+  P->dt = c->dt;
+  P->half_dt = 0.5 * c->dt;
+  P->m_s = c->m_s;
+  P->v_T = c->v_T;
+  P->v_fl = c->v_fl;
+  P->lam = c->r_D_e;
+  P->rho0 = c->r_0 / c->r_D_e;
+  const double C = c->kqq / (c->r_D_e * c->r_D_e);  // kJ/mol/nm
+  if (c->precision == OD_PREC_FP32C) {
+    P->pos_scale = L / c->r_D_e;
+    P->pair_scale = -C * L * L;
+  } else {
+    P->pos_scale = 1.0 / c->r_D_e;
+    P->pair_scale = -C;
+  }
+  P->a32 = (float)(P->rho0 * L);
+  P->linv32 = (float)(1.0 / L);
+  P->thr_top = c->thr_top;
+  P->thr_side = c->use_bottom ? c->thr_side : 4294967296ull;
+  P->thr_coll = c->thr_coll;
+  ExtParams& E = P->ext;
+  E.n_dust = c->n_dust;
+  E.field_driven = c->distribution == OD_DIST_FIELD_DRIVEN;
+  E.inv_lam = 1.0 / c->r_D_e;
+  E.trap_scale = -c->trap_C / c->r_D_e;
+  for (int m = 0; m < 15; ++m) E.alpha[m] = c->trap_alpha[m];
+  E.k_dust = c->k_dust;
+  E.field_force = c->field_force;
+  for (int p = 0; p < c->n_dust; ++p) {
+    E.dx[p] = c->dust_x[p];
+    E.dy[p] = c->dust_y[p];
+    E.dz[p] = c->dust_z[p];
+    E.dr[p] = c->dust_r[p];
+    P->dust_method[p] = c->dust_method[p];
+    P->si_dx[p] = c->si_dust_x[p];
+    P->si_dy[p] = c->si_dust_y[p];
+    P->si_dz[p] = c->si_dust_z[p];
+    P->si_dr[p] = c->si_dust_r[p];
+  }
+  P->si_k = c->si_k;
+  P->si_e = c->si_e;
+  P->si_q_s = c->si_q_s;
+  P->si_lam = c->si_r_D_e;
+  P->si_dt = c->si_dt;
+  P->si_kB = c->si_kB;
+  P->si_Te = c->si_Te;
+  P->si_Tn = c->si_Tn;
+  P->si_E = c->si_E;
+  P->gamma_e_pref = c->gamma_e_pref;
+  P->oml_A = c->oml_A;
+  P->oml_B = c->oml_B;
+  P->oml_u2 = c->oml_u2;
+  P->oml_erf_u = c->oml_erf_u;
+  P->oml_tail = c->oml_tail;
+  P->collect_scale = c->collect_scale;
+  P->top = c->top;
+  P->bottom = c->bottom;
+  P->side = c->side;
+  P->vz = c->vz;
+  return OD_OK;
+}
+
+static void free_ctx(od_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->dev);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
+                  c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
+                  c->slot_of, c->list_ion, c->err, c->rec};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <class P>
+static int dalloc(P** p, size_t bytes) {
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return OD_OK;
+}
+
+// launch geometry of the pair kernel
+static void pair_geometry(int n_pad, int ib, int j_split, int* n_chunks, int* chunk_len) {
+  const int n_iblk = n_pad / (256 * ib);
+  int c = j_split;
+  if (c <= 0) {
+    // enough blocks to fill 256 CUs several times, chunks of >= 1024 sources
+    c = 1;
+    while (n_iblk * c < 4096 && c < 64 && n_pad / (2 * c) >= 1024) c *= 2;
+  }
+  c = std::max(1, std::min(c, n_pad / TJ));
+  int len = (n_pad + c - 1) / c;
+  len = (len + TJ - 1) / TJ * TJ;
+  *n_chunks = c;
+  *chunk_len = len;
+}
+
+static int check_config(const od_config* c) {
+  if (!c) return set_err(OD_ERR_ARG, "null config");
+  if (c->abi_version != OD_ABI_VERSION)
+    return set_err(OD_ERR_ARG, "ABI version %d, library expects %d", c->abi_version, OD_ABI_VERSION);
+  if (c->n_ions < 1) return set_err(OD_ERR_ARG, "n_ions must be >= 1 (got %d)", c->n_ions);
+  if (c->n_ions > (1 << 28)) return set_err(OD_ERR_ARG, "n_ions %d above the supported 2^28", c->n_ions);
+  if (c->n_dust < 0 || c->n_dust > OD_MAX_DUST)
+    return set_err(OD_ERR_ARG, "n_dust %d outside [0, %d]", c->n_dust, OD_MAX_DUST);
+  if (c->precision != OD_PREC_FP32C && c->precision != OD_PREC_FP64)
+    return set_err(OD_ERR_ARG, "unknown precision %d", c->precision);
+  if (c->distribution != OD_DIST_MAXWELLIAN && c->distribution != OD_DIST_FIELD_DRIVEN)
+    return set_err(OD_ERR_ARG, "unknown distribution %d", c->distribution);
+  const od_table* ts[4] = {&c->top, &c->bottom, &c->side, &c->vz};
+  for (const od_table* t : ts)
+    if (t->n_seg < 0 || t->n_seg > OD_MAX_TABLE) return set_err(OD_ERR_ARG, "table length %d invalid", t->n_seg);
+  if (!(c->R > 0 && c->H > 0 && c->dt > 0 && c->r_D_e > 0 && c->m_s > 0))
+    return set_err(OD_ERR_ARG, "R, H, dt, r_D_e, m_s must be positive");
+  return OD_OK;
+}
+
+extern "C" int od_create(const od_config* cfg, od_ctx** out) {
+  if (!out) return set_err(OD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  CHKRET(check_config(cfg));
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return set_err(OD_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return set_err(OD_ERR_ARG, "device %d outside [0, %d)", cfg->device, ndev);
+  od_ctx* c = new od_ctx();
+  c->cfg = *cfg;
+  c->dev = cfg->device;
+  c->prec = cfg->precision;
+  c->n = cfg->n_ions;
+  c->n_pad = (c->n + 1023) / 1024 * 1024;
+  c->nb = c->n_pad / 256;
+  c->ib = (c->prec == OD_PREC_FP32C) ? 2 : 1;
+  pair_geometry(c->n_pad, c->ib, cfg->j_split, &c->n_chunks, &c->chunk_len);
+  c->pos_bytes = (c->prec == OD_PREC_FP32C) ? sizeof(float4) : sizeof(double4);
+  c->rows_grid = 1024;
+  int rc = OD_OK;
+  auto fail = [&](int r) {
+    free_ctx(c);
+    return r;
+  };
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice(%d) failed", c->dev));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "hipEventCreate failed"));
+  const size_t np = c->n_pad;
+  if ((rc = dalloc(&c->dP, sizeof(DevParams))) || (rc = dalloc(&c->X, 3 * np * 8)) ||
+      (rc = dalloc(&c->V, 3 * np * 8)) || (rc = dalloc(&c->F, 3 * np * 8)) ||
+      (rc = dalloc(&c->Fpart, (c->n_chunks > 1 ? c->n_chunks : 1) * 3 * np * 8)) ||
+      (rc = dalloc(&c->q, OD_MAX_DUST * 8)) || (rc = dalloc(&c->partials, (size_t)c->nb * OD_MAX_DUST * 8 * 8)) ||
+      (rc = dalloc(&c->rowF, 3 * np * 8)) || (rc = dalloc(&c->pi, np * c->pos_bytes)) ||
+      (rc = dalloc(&c->pj, np * c->pos_bytes)) || (rc = dalloc(&c->list_old, np * c->pos_bytes)) ||
+      (rc = dalloc(&c->list_new, np * c->pos_bytes)) || (rc = dalloc(&c->flags, np * 4)) ||
+      (rc = dalloc(&c->st, 4 * 4)) || (rc = dalloc(&c->face, np)) || (rc = dalloc(&c->block_cnt, c->nb * 4)) ||
+      (rc = dalloc(&c->block_off, c->nb * 4)) || (rc = dalloc(&c->n_flag, 4)) ||
+      (rc = dalloc(&c->slot_of, np * 4)) || (rc = dalloc(&c->list_ion, np * 4)) || (rc = dalloc(&c->err, 4)) ||
+      (rc = dalloc(&c->rec, (size_t)c->rec_cap * std::max(1, cfg->n_dust) * sizeof(od_dust_record))))
+    return fail(rc);
+  DevParams hp;
+  fill_params(cfg, c->n_pad, &hp);
+  double q0[OD_MAX_DUST] = {0};
+  for (int p = 0; p < cfg->n_dust; ++p) q0[p] = cfg->dust_q0[p];
+  uint32_t st0[4] = {0, 0, 0, 0};
+  if (hipMemcpy(c->dP, &hp, sizeof(hp), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->q, q0, sizeof(q0), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->st, st0, sizeof(st0), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->X, 0, 3 * np * 8) != hipSuccess || hipMemset(c->V, 0, 3 * np * 8) != hipSuccess ||
+      hipMemset(c->F, 0, 3 * np * 8) != hipSuccess || hipMemset(c->err, 0, 4) != hipSuccess ||
+      hipMemset(c->n_flag, 0, 4) != hipSuccess || hipMemset(c->flags, 0, np * 4) != hipSuccess ||
+      hipMemset(c->face, 0, np) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "device initialisation failed"));
+  *out = c;
+  return OD_OK;
+}
+
+extern "C" int od_destroy(od_ctx* c) {
+  free_ctx(c);
+  return OD_OK;
+}
+
+// pair force of the current (pi, pj) into F, optionally timed
+static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, bool timed) {
+  const DevParams* dP = c->dP;
+  (void)dP;
+  double* out = (c->n_chunks > 1) ? c->Fpart : F;
+  const int n_iblk = c->n_pad / (256 * c->ib);
+  const int grid = n_iblk * c->n_chunks;
+  const double L = 1.4426950408889634;
+  const double C = c->cfg.kqq / (c->cfg.r_D_e * c->cfg.r_D_e);
+  if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (c->prec == OD_PREC_FP32C) {
+    const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
+    hipLaunchKernelGGL(k_pair32<2>, dim3(grid), dim3(256), 0, c->stream, (const float4*)pi, (const float4*)pj,
+                       c->n_pad, c->n_chunks, c->chunk_len, a, linv, -C * L * L, out);
+  } else {
+    hipLaunchKernelGGL(k_pair64<1>, dim3(grid), dim3(256), 0, c->stream, (const double4*)pi, (const double4*)pj,
+                       c->n_pad, c->n_chunks, c->chunk_len, c->cfg.r_0 / c->cfg.r_D_e, -C, out);
+  }
+  if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+  if (c->n_chunks > 1) {
+    const int n3 = 3 * c->n_pad;
+    hipLaunchKernelGGL(k_chunk_reduce, dim3((n3 + 255) / 256), dim3(256), 0, c->stream, c->Fpart, c->n_chunks, n3,
+                       F);
+  }
+  return OD_OK;
+}
+
+template <class T>
+static int launch_step_t(od_ctx* c, bool timed) {
+  using PV = typename Pos<T>::V;
+  const int nb = c->nb;
+  hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
+                     (PV*)c->pi, (PV*)c->pj, c->flags, c->block_cnt, c->err);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->block_cnt, nb, c->block_off, c->n_flag);
+  hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, c->stream, c->flags, c->block_cnt, c->block_off, c->n,
+                     (const PV*)c->pi, c->slot_of, c->list_ion, (PV*)c->list_old);
+  CHKRET(launch_pair(c, c->pi, c->pj, c->F, timed));
+  hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
+                     (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
+                     (PV*)c->list_new, c->face, c->partials);
+  hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pj, c->n_flag,
+                     (const PV*)c->list_new, c->rowF);
+  hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->F, (const PV*)c->pj, c->flags,
+                     c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
+  hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, nb, c->q, c->st, c->rec);
+  return OD_OK;
+}
+
+static int launch_step(od_ctx* c, bool timed) {
+  return (c->prec == OD_PREC_FP32C) ? launch_step_t<float>(c, timed) : launch_step_t<double>(c, timed);
+}
+
+template <class T>
+static int load_positions_t(od_ctx* c, const double* X, void* pi, void* pj) {
+  using PV = typename Pos<T>::V;
+  const double s = (c->prec == OD_PREC_FP32C) ? 1.4426950408889634 / c->cfg.r_D_e : 1.0 / c->cfg.r_D_e;
+  hipLaunchKernelGGL(k_load_pos<T>, dim3(c->nb), dim3(256), 0, c->stream, X, c->n, c->n_pad, s, (PV*)pi, (PV*)pj);
+  return OD_OK;
+}
+
+static int load_positions(od_ctx* c, const double* X, void* pi, void* pj) {
+  return (c->prec == OD_PREC_FP32C) ? load_positions_t<float>(c, X, pi, pj) : load_positions_t<double>(c, X, pi, pj);
+}
+
+static void aos_to_soa(const double* a, int n, int n_pad, std::vector<double>& s) {
+  s.assign(3 * (size_t)n_pad, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) s[(size_t)k * n_pad + i] = a[3 * (size_t)i + k];
+}
+
+extern "C" int od_set_ions(od_ctx* c, const double* x, const double* v, int64_t n) {
+  if (!c || !x || !v) return set_err(OD_ERR_ARG, "null argument");
+  if (n != c->n) return set_err(OD_ERR_ARG, "od_set_ions: %lld ions, context holds %d", (long long)n, c->n);
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(v[i]))
+      return set_err(OD_ERR_NONFINITE, "od_set_ions: non-finite value at ion %lld", (long long)(i / 3));
+  HIPCHK(hipSetDevice(c->dev));
+  std::vector<double> sx, sv;
+  aos_to_soa(x, c->n, c->n_pad, sx);
+  aos_to_soa(v, c->n, c->n_pad, sv);
+  HIPCHK(hipMemcpyAsync(c->X, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->V, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, c->stream));
+  CHKRET(load_positions(c, c->X, c->pi, c->pj));
+  CHKRET(launch_pair(c, c->pi, c->pj, c->F, false));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OD_OK;
+}
+
+extern "C" int od_get_ions(od_ctx* c, double* x, double* v) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  std::vector<double> s(3 * (size_t)c->n_pad);
+  if (x) {
+    HIPCHK(hipMemcpyAsync(s.data(), c->X, s.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->n; ++i)
+      for (int k = 0; k < 3; ++k) x[3 * (size_t)i + k] = s[(size_t)k * c->n_pad + i];
+  }
+  if (v) {
+    HIPCHK(hipMemcpyAsync(s.data(), c->V, s.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->n; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * (size_t)i + k] = s[(size_t)k * c->n_pad + i];
+  }
+  return OD_OK;
+}
+
+extern "C" int od_set_dust_charges(od_ctx* c, const double* q) {
+  if (!c || !q) return set_err(OD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  double h[OD_MAX_DUST] = {0};
+  for (int p = 0; p < c->cfg.n_dust; ++p) h[p] = q[p];
+  HIPCHK(hipMemcpyAsync(c->q, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OD_OK;
+}
+
+extern "C" int od_get_dust_charges(od_ctx* c, double* q) {
+  if (!c || !q) return set_err(OD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  double h[OD_MAX_DUST];
+  HIPCHK(hipMemcpyAsync(h, c->q, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int p = 0; p < c->cfg.n_dust; ++p) q[p] = h[p];
+  return OD_OK;
+}
+
+extern "C" int od_set_step(od_ctx* c, int64_t step) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (step < 0 || step > 0xFFFFFFFFll) return set_err(OD_ERR_ARG, "step %lld outside u32 range", (long long)step);
+  HIPCHK(hipSetDevice(c->dev));
+  uint32_t s = (uint32_t)step;
+  HIPCHK(hipMemcpyAsync(c->st, &s, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->host_step = step;
+  return OD_OK;
+}
+
+static int build_graph(od_ctx* c, int steps) {
+  if (c->gexec && c->graph_steps == steps) return OD_OK;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = OD_OK;
+  for (int s = 0; s < steps && rc == OD_OK; ++s) rc = launch_step(c, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc != OD_OK) return rc;
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  c->graph = g;
+  HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  c->graph_steps = steps;
+  return OD_OK;
+}
+
+extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t flags) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (nsteps < 0) return set_err(OD_ERR_ARG, "negative step count");
+  if (c->host_step + nsteps > 0xFFFFFFFFll) return set_err(OD_ERR_ARG, "step counter would exceed u32");
+  HIPCHK(hipSetDevice(c->dev));
+  const bool profile = (flags & OD_RUN_PROFILE_PAIR) != 0;
+  const bool graph = (flags & OD_RUN_GRAPH) != 0 && !profile;
+  const int nd = c->cfg.n_dust;
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int chunk = (int)std::min<int64_t>(c->rec_cap, nsteps - done);
+    uint32_t base = (uint32_t)c->host_step;
+    HIPCHK(hipMemcpyAsync(c->st + 1, &base, 4, hipMemcpyHostToDevice, c->stream));
+    int left = chunk;
+    if (graph && chunk >= 4) {
+      const int gs = std::min(chunk, 32);
+      CHKRET(build_graph(c, gs));
+      while (left >= gs) {
+        HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        left -= gs;
+      }
+    }
+    for (; left > 0; --left) {
+      CHKRET(launch_step(c, profile));
+      HIPCHK(hipGetLastError());
+      if (profile) {
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->pair_ms += ms;
+        c->pair_launches += 1;
+      }
+    }
+    if (out && nd > 0)
+      HIPCHK(hipMemcpyAsync(out + done * nd, c->rec, (size_t)chunk * nd * sizeof(od_dust_record),
+                            hipMemcpyDeviceToHost, c->stream));
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->host_step += chunk;
+    done += chunk;
+    if (herr) return set_err(OD_ERR_NONFINITE, "non-finite ion position or velocity before step %lld",
+                             (long long)c->host_step);
+  }
+  return OD_OK;
+}
+
+extern "C" int od_pair_timing(od_ctx* c, double* total_ms, int64_t* launches, int32_t reset) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (total_ms) *total_ms = c->pair_ms;
+  if (launches) *launches = c->pair_launches;
+  if (reset) {
+    c->pair_ms = 0.0;
+    c->pair_launches = 0;
+  }
+  return OD_OK;
+}
+
+extern "C" int od_get_step_info(od_ctx* c, uint32_t* flags, uint8_t* face, int64_t* n_reinjected) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (flags) HIPCHK(hipMemcpyAsync(flags, c->flags, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (face) HIPCHK(hipMemcpyAsync(face, c->face, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  int nf = 0;
+  HIPCHK(hipMemcpyAsync(&nf, c->n_flag, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (n_reinjected) *n_reinjected = nf;
+  return OD_OK;
+}
+
+extern "C" int od_pair_forces(od_ctx* c, const double* x, double* f) {
+  if (!c || !x || !f) return set_err(OD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t np = c->n_pad;
+  double *dX = nullptr, *dF = nullptr;
+  void *pi = nullptr, *pj = nullptr;
+  int rc = OD_OK;
+  if ((rc = dalloc(&dX, 3 * np * 8)) || (rc = dalloc(&dF, 3 * np * 8)) || (rc = dalloc(&pi, np * c->pos_bytes)) ||
+      (rc = dalloc(&pj, np * c->pos_bytes))) {
+    hipFree(dX);
+    hipFree(dF);
+    hipFree(pi);
+    hipFree(pj);
+    return rc;
+  }
+  std::vector<double> sx;
+  aos_to_soa(x, c->n, c->n_pad, sx);
+  std::vector<double> sf(3 * np);
+  hipError_t e = hipMemcpyAsync(dX, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    rc = load_positions(c, dX, pi, pj);
+    if (rc == OD_OK) rc = launch_pair(c, pi, pj, dF, false);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(sf.data(), dF, sf.size() * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  hipFree(dX);
+  hipFree(dF);
+  hipFree(pi);
+  hipFree(pj);
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_pair_forces: %s", hipGetErrorString(e));
+  if (rc != OD_OK) return rc;
+  for (int i = 0; i < c->n; ++i)
+    for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] = sf[(size_t)k * np + i];
+  return OD_OK;
+}
+
+extern "C" int od_trap_potential_grid(int32_t device, double R, double H, int32_t n_r, const double* rho,
+                                      int32_t n_h, const double* h, int64_t n_mc, uint64_t seed, double* out) {
+  if (!rho || !h || !out) return set_err(OD_ERR_ARG, "null argument");
+  if (n_r < 1 || n_h < 1 || n_mc < 1) return set_err(OD_ERR_ARG, "grid and sample counts must be positive");
+  if ((int64_t)n_r * n_h > (1 << 22)) return set_err(OD_ERR_ARG, "trap grid too large");
+  int ndev = 0;
+  hipError_t e0 = hipGetDeviceCount(&ndev);
+  if (e0 != hipSuccess || ndev == 0) return set_err(OD_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_err(OD_ERR_ARG, "device %d outside [0, %d)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  const int n_gp = n_r * n_h;
+  const int64_t chunk = 1 << 16;
+  const int n_chunks = (int)((n_mc + chunk - 1) / chunk);
+  if ((int64_t)n_gp * n_chunks > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "trap grid x samples too large");
+  double *xs = nullptr, *ys = nullptr, *zs = nullptr, *drho = nullptr, *dh = nullptr, *part = nullptr,
+         *dout = nullptr;
+  int rc = OD_OK;
+  auto cleanup = [&]() {
+    hipFree(xs);
+    hipFree(ys);
+    hipFree(zs);
+    hipFree(drho);
+    hipFree(dh);
+    hipFree(part);
+    hipFree(dout);
+  };
+  if ((rc = dalloc(&xs, n_mc * 8)) || (rc = dalloc(&ys, n_mc * 8)) || (rc = dalloc(&zs, n_mc * 8)) ||
+      (rc = dalloc(&drho, n_r * 8)) || (rc = dalloc(&dh, n_h * 8)) ||
+      (rc = dalloc(&part, (size_t)n_gp * n_chunks * 8)) || (rc = dalloc(&dout, (size_t)n_gp * 8))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpy(drho, rho, n_r * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dh, h, n_h * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_trap_samples, dim3((unsigned)((n_mc + 255) / 256)), dim3(256), 0, 0, R, H, seed, n_mc, xs,
+                       ys, zs);
+    hipLaunchKernelGGL(k_trap_grid, dim3(n_gp * n_chunks), dim3(256), 0, 0, drho, dh, n_h, xs, ys, zs, n_mc, chunk,
+                       n_chunks, part);
+    hipLaunchKernelGGL(k_trap_finish, dim3((n_gp + 255) / 256), dim3(256), 0, 0, part, n_gp, n_chunks, (double)n_mc,
+                       dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n_gp * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  cleanup();
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_trap_potential_grid: %s", hipGetErrorString(e));
+  return OD_OK;
+}
+
+extern "C" int od_trap_samples_host(double R, double H, uint64_t seed, int64_t k0, int64_t n, double* xyz) {
+  if (!xyz || n < 0 || k0 < 0) return set_err(OD_ERR_ARG, "bad argument");
+  for (int64_t k = 0; k < n; ++k) trap_sample(R, H, seed, k0 + k, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]);
+  return OD_OK;
+}
+
+extern "C" int od_abi_sizes(int64_t* config_size, int64_t* record_size) {
+  if (config_size) *config_size = (int64_t)sizeof(od_config);
+  if (record_size) *record_size = (int64_t)sizeof(od_dust_record);
+  return OD_OK;
+}
+
+extern "C" const char* od_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950, ABI " "1"; }

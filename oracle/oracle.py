@@ -1,0 +1,190 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/od_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the engine.  Build with
+``make -C oracle`` (or ``oracle.build()``).
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    from opendust_amd.engine import Config, DustRecord, Table  # structure layouts only
+
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    dp = P(ctypes.c_double)
+    lib.orc_philox.argtypes = [P(ctypes.c_uint32), ctypes.c_uint64, P(ctypes.c_uint32)]
+    lib.orc_table_eval.argtypes = [P(Table), ctypes.c_double]
+    lib.orc_table_eval.restype = ctypes.c_double
+    lib.orc_pair_forces.argtypes = [P(Config), ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_ext_force.argtypes = [P(Config), dp, dp, dp]
+    lib.orc_reinject.argtypes = [P(Config), ctypes.c_uint32, ctypes.c_uint32, dp, dp]
+    lib.orc_reinject.restype = ctypes.c_int
+    lib.orc_collide.argtypes = [P(Config), ctypes.c_uint32, ctypes.c_uint32, dp]
+    lib.orc_flags.argtypes = [P(Config), dp]
+    lib.orc_flags.restype = ctypes.c_uint32
+    lib.orc_orbit_term.argtypes = [P(Config), ctypes.c_int, dp, dp]
+    lib.orc_charge.argtypes = [P(Config), ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_int64]
+    lib.orc_charge.restype = ctypes.c_double
+    lib.orc_out_domain_force.argtypes = [P(Config), ctypes.c_int, ctypes.c_double, dp]
+    lib.orc_step.argtypes = [P(Config), ctypes.c_int, dp, dp, dp, ctypes.c_int64, P(DustRecord),
+                             P(ctypes.c_uint32), P(ctypes.c_uint8), ctypes.c_int]
+    lib.orc_run.argtypes = [P(Config), ctypes.c_int, dp, dp, dp, ctypes.c_int64, ctypes.c_int64, P(DustRecord),
+                            ctypes.c_int]
+    lib.orc_trap_sample.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64, dp]
+    lib.orc_trap_grid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_int, dp,
+                                  ctypes.c_int64, ctypes.c_uint64, dp, ctypes.c_int]
+    lib.orc_max_threads.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def threads():
+    return load().orc_max_threads()
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (ctypes.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    o = (ctypes.c_uint32 * 4)()
+    lib.orc_philox(c, int(key) & (2 ** 64 - 1), o)
+    return [int(v) for v in o]
+
+
+def table_eval(table_struct, u):
+    return load().orc_table_eval(ctypes.byref(table_struct), float(u))
+
+
+def pair_forces(cfg, x, rows=None, nthreads=0):
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    i0, i1 = (0, n) if rows is None else rows
+    f = np.empty((i1 - i0, 3))
+    lib.orc_pair_forces(ctypes.byref(cfg), n, _d(x), _d(f), i0, i1, nthreads or threads())
+    return f
+
+
+def ext_force(cfg, q, xi):
+    lib = load()
+    q = np.ascontiguousarray(np.resize(np.asarray(q, dtype=np.float64), max(cfg.n_dust, 1)))
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    f = np.empty(3)
+    lib.orc_ext_force(ctypes.byref(cfg), _d(q), _d(xi), _d(f))
+    return f
+
+
+def reinject(cfg, ion, step):
+    lib = load()
+    x = np.zeros(3)
+    v = np.zeros(3)
+    face = lib.orc_reinject(ctypes.byref(cfg), ion, step, _d(x), _d(v))
+    return face, x, v
+
+
+def collide(cfg, ion, step, v):
+    v = np.array(v, dtype=np.float64)
+    load().orc_collide(ctypes.byref(cfg), ion, step, _d(v))
+    return v
+
+
+def flags(cfg, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return int(load().orc_flags(ctypes.byref(cfg), _d(x)))
+
+
+def orbit_term(cfg, p, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    o = np.empty(3)
+    load().orc_orbit_term(ctypes.byref(cfg), p, _d(x), _d(o))
+    return o
+
+
+def charge(cfg, p, step, q_prev, absorbed):
+    return load().orc_charge(ctypes.byref(cfg), p, int(step), float(q_prev), int(absorbed))
+
+
+def out_domain_force(cfg, p, qp):
+    f = np.empty(3)
+    load().orc_out_domain_force(ctypes.byref(cfg), p, float(qp), _d(f))
+    return f
+
+
+def step(cfg, x, v, q, step_index, nthreads=0):
+    """One oracle step in place on copies; returns (x, v, q, records, flags, faces)."""
+    from opendust_amd.engine import DustRecord, RECORD_DTYPE
+
+    lib = load()
+    x = np.array(x, dtype=np.float64).reshape(-1, 3)
+    v = np.array(v, dtype=np.float64).reshape(-1, 3)
+    q = np.array(np.resize(np.asarray(q, dtype=np.float64), max(cfg.n_dust, 1)))
+    n = x.shape[0]
+    rec = np.zeros(max(cfg.n_dust, 1), dtype=RECORD_DTYPE)
+    fl = np.zeros(n, dtype=np.uint32)
+    face = np.zeros(n, dtype=np.uint8)
+    rc = lib.orc_step(ctypes.byref(cfg), n, _d(x), _d(v), _d(q), int(step_index),
+                      rec.ctypes.data_as(ctypes.POINTER(DustRecord)),
+                      fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                      face.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), nthreads or threads())
+    if rc:
+        raise RuntimeError("orc_step failed")
+    return x, v, q[: cfg.n_dust], rec[: cfg.n_dust], fl, face
+
+
+def run(cfg, x, v, q, step0, nsteps, nthreads=0):
+    """nsteps oracle steps; returns (x, v, q, records[nsteps][n_dust])."""
+    from opendust_amd.engine import DustRecord, RECORD_DTYPE
+
+    lib = load()
+    x = np.array(x, dtype=np.float64).reshape(-1, 3)
+    v = np.array(v, dtype=np.float64).reshape(-1, 3)
+    q = np.array(np.resize(np.asarray(q, dtype=np.float64), max(cfg.n_dust, 1)))
+    rec = np.zeros((nsteps, max(cfg.n_dust, 1)), dtype=RECORD_DTYPE)
+    rc = lib.orc_run(ctypes.byref(cfg), x.shape[0], _d(x), _d(v), _d(q), int(step0), int(nsteps),
+                     rec.ctypes.data_as(ctypes.POINTER(DustRecord)), nthreads or threads())
+    if rc:
+        raise RuntimeError("orc_run failed")
+    return x, v, q[: cfg.n_dust], rec[:, : cfg.n_dust]
+
+
+def trap_sample(R, H, seed, k):
+    o = np.empty(3)
+    load().orc_trap_sample(float(R), float(H), int(seed) & (2 ** 64 - 1), int(k), _d(o))
+    return o
+
+
+def trap_grid(R, H, rho, h, n_mc, seed, nthreads=0):
+    lib = load()
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    out = np.empty((len(rho), len(h)))
+    rc = lib.orc_trap_grid(float(R), float(H), len(rho), _d(rho), len(h), _d(h), int(n_mc),
+                           int(seed) & (2 ** 64 - 1), _d(out), nthreads or threads())
+    if rc:
+        raise RuntimeError("orc_trap_grid failed")
+    return out

@@ -1,0 +1,183 @@
+"""Engine (HIP, through the C ABI) against the FP64 CPU oracle on identical
+seeded states.
+
+Tolerances (stated here, checked below):
+  * FP64 engine vs oracle: pair forces per ion rel <= 1e-11; one step x, v
+    rel <= 1e-11 (positions relative to the domain size); dust records rel
+    <= 1e-9.
+  * FP32-compensated engine vs FP64 oracle.  Error model: positions enter the
+    pair kernel rounded to FP32 in units of r_D_e/log2(e) (|u| <= ~10), so a
+    pair at distance r carries a relative error ~ ulp(|u|)/r_u ~ 1e-6/r_u on
+    top of a few FP32 ulps of the arithmetic; the compensated sums add nothing
+    visible.  Close pairs dominate the force, hence: pair forces norm-wise rel
+    <= 1e-5, per ion |dF_i| <= 1e-4 |F_i| + 1e-5 rms|F|; after one step
+    |dx| <= 1e-8 R, |dv| <= 1e-4 max|v|, dust records rel <= 1e-5 of their
+    scale.
+  * Integer/index outputs (flags, reinjection faces, absorbed counts) are
+    bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine_mod():
+    from opendust_amd import engine
+
+    engine.load_library()
+    return engine
+
+
+def _setup(N=2048, precision="fp64", seed=7, np_seed=3, workload="single_dust", craft=True):
+    from opendust_amd import workloads
+
+    od = getattr(workloads, workload)(N=N, n=100, precision=precision, seed=seed)
+    alpha = workloads.synthetic_alpha(od)
+    cfg, x0, v0 = workloads.prepare(od, alpha=alpha, np_seed=np_seed, precision=precision)
+    if craft:
+        # ions 0..5 inside the dust(s), 6..11 outside the side wall, 12..17
+        # beyond the end caps: they are absorbed / flagged this step
+        rng = np.random.RandomState(11)
+        nd = cfg.n_dust
+        for i in range(6):
+            p = i % max(nd, 1)
+            if nd:
+                d = rng.normal(size=3)
+                d *= 0.5 * cfg.dust_r[p] / np.linalg.norm(d)
+                x0[i] = [cfg.dust_x[p] + d[0], cfg.dust_y[p] + d[1], cfg.dust_z[p] + d[2]]
+        for i in range(6, 12):
+            a = rng.uniform(0, 2 * np.pi)
+            x0[i] = [1.02 * cfg.R * np.cos(a), 1.02 * cfg.R * np.sin(a), rng.uniform(-0.4, 0.4) * cfg.H]
+        for i in range(12, 18):
+            x0[i, 2] = (0.5 + 0.01) * cfg.H * (1 if i % 2 else -1)
+    return od, cfg, x0, v0
+
+
+def test_library_loads_on_gpu():
+    eng = _engine_mod()
+    assert eng.check_abi()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_pair_forces_vs_oracle(oracle_lib, precision):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=3000, precision=precision, craft=False)
+    say("pair forces", precision, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    f_gpu = e.pair_forces(x0)
+    e.close()
+    f_orc = oracle_lib.pair_forces(cfg, x0)
+    fn = np.linalg.norm(f_orc, axis=1)
+    err = np.linalg.norm(f_gpu - f_orc, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    norm_rel = np.linalg.norm(f_gpu - f_orc) / np.linalg.norm(f_orc)
+    say("  norm-wise rel", norm_rel, "max per-ion rel", np.max(err / fn))
+    if precision == "fp64":
+        assert np.all(err <= 1e-11 * fn + 1e-14 * rms)
+    else:
+        assert norm_rel <= 1e-5
+        assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_one_step_vs_oracle(oracle_lib, precision):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=2048, precision=precision)
+    q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
+    say("one step", precision, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    e.set_step(5)
+    rec = e.run(1)
+    x_g, v_g = e.get_ions()
+    fl_g, face_g, nre = e.step_info()
+    e.close()
+    x_o, v_o, q_o, rec_o, fl_o, face_o = oracle_lib.step(cfg, x0, v0, q0, 5)
+    assert np.array_equal(fl_g, fl_o)
+    assert np.array_equal(face_g, face_o)
+    assert nre == int((fl_o != 0).sum()) and nre >= 18
+    assert np.array_equal(rec["absorbed"][0], rec_o["absorbed"])
+    dx = np.max(np.abs(x_g - x_o)) / cfg.R
+    dv = np.max(np.abs(v_g - v_o)) / np.max(np.abs(v_o))
+    say("  max |dx|/R", dx, "max |dv|/max|v|", dv)
+    assert dx <= (1e-11 if precision == "fp64" else 1e-8)
+    assert dv <= (1e-10 if precision == "fp64" else 1e-4)
+    tol_r = 1e-9 if precision == "fp64" else 1e-5
+    for key in ("q", "force_orbit", "force_collect", "force_dust"):
+        a = np.asarray(rec[key][0], dtype=np.float64)
+        b = np.asarray(rec_o[key], dtype=np.float64)
+        sc = max(np.max(np.abs(b)), 1e-300)
+        say("  record", key, "rel", np.max(np.abs(a - b)) / sc)
+        assert np.max(np.abs(a - b)) <= tol_r * sc, key
+
+
+def test_multi_step_fp64_records(oracle_lib):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=1024, precision="fp64", craft=True)
+    q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
+    say("multi step fp64 N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    rec = e.run(6)
+    x_g, v_g = e.get_ions()
+    e.close()
+    x_o, v_o, q_o, rec_o = oracle_lib.run(cfg, x0, v0, q0, 0, 6)
+    assert np.array_equal(rec["absorbed"], rec_o["absorbed"])
+    assert np.max(np.abs(rec["q"] - rec_o["q"]) / np.abs(rec_o["q"])) <= 1e-12
+    for key in ("force_orbit", "force_collect"):
+        sc = np.max(np.abs(rec_o[key]))
+        assert np.max(np.abs(rec[key] - rec_o[key])) <= 1e-9 * sc, key
+    assert np.max(np.abs(x_g - x_o)) <= 1e-10 * cfg.R
+
+
+def test_determinism_and_graph(oracle_lib):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=4096, precision="fp32c", craft=True)
+    say("determinism fp32c N", cfg.n_ions)
+    outs = []
+    for flags in (0, 0, eng.RUN_GRAPH):
+        e = eng.Engine(cfg)
+        e.set_ions(x0, v0)
+        rec = e.run(40, flags)
+        x, v = e.get_ions()
+        e.close()
+        outs.append((rec, x, v))
+    for rec, x, v in outs[1:]:
+        assert np.array_equal(rec, outs[0][0])
+        assert np.array_equal(x, outs[0][1])
+        assert np.array_equal(v, outs[0][2])
+
+
+def test_trap_grid_vs_oracle(oracle_lib):
+    eng = _engine_mod()
+    R, H = 5.0, 10.0
+    rho = np.linspace(0, R, 6)
+    h = np.linspace(0, H / 2, 5)
+    say("trap grid")
+    g = eng.trap_potential_grid(R, H, rho, h, 200000, 99)
+    o = oracle_lib.trap_grid(R, H, rho, h, 200000, 99)
+    assert np.max(np.abs(g - o) / np.abs(o)) <= 1e-12
+
+
+def test_simulate_api_small(tmp_path):
+    from opendust_amd import workloads
+
+    od = workloads.single_dust(N=4096, n=30, trapSamples=200000, trapSeed=5)
+    od.outputParameters.nFileOutput = 10
+    od.outputParameters.xyzOutputFileName = str(tmp_path / "traj.xyz")
+    od.outputParameters.restartFileName = str(tmp_path / "RESTART")
+    np.random.seed(0)
+    say("simulate small")
+    od.simulate()
+    d = od.dustParticles[0]
+    assert d.q.shape == (30,) and d.forceIonsOrbit.shape == (30, 3)
+    assert np.all(np.isfinite(d.forceIonsOrbit)) and np.all(np.isfinite(d.q))
+    assert d.q[0] == d.q[0] and d.q[1] != d.q[0]
+    lines = open(tmp_path / "traj.xyz").read().splitlines()
+    assert lines[0] == "4096" and len(lines) == 3 * (4096 + 2)
+    rs = open(tmp_path / "RESTART").read().splitlines()
+    assert len(rs) == 4096 and len(rs[0].split("\t")) == 6
