@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Headline benchmark: pair interactions/s and steps/s of the ion-flow step at
+2^20 superions (single dust, all-pairs), FP32-compensated pair arithmetic,
+with the plain FP64 GPU path and the FP64 CPU oracle timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line on stdout; progress goes to stderr.
+
+A step is one full reference step (opendust.py:1059-1384: collisions, KDK,
+flags, absorbed counts and momenta, reinjection, orbit sums, charge update,
+dust forces) over all N ions, with the ion state resident on the GPU.
+pair interactions/s = N (N - 1) x steps/s.
+
+Extra modes (used for the committed profiles):
+    --copy-calibration   plain device copies of known size (rocprof PMC scale)
+    --pair-only K        K launches of the main pair kernel only
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector with packed ops; also matrix), TFLOP/s
+FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
+METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def copy_calibration(nbytes=1 << 30, reps=8):
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
+def cpu_baseline(cfg, x, target_s=12.0):
+    """FP64 oracle pair-force rows against all N ions, all host threads."""
+    from oracle import oracle
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.threads()
+    n = x.shape[0]
+    rows = 64
+    t0 = time.perf_counter()
+    oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
+    t1 = time.perf_counter() - t0
+    rows = int(min(n, max(64, rows * target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    pairs = rows * (n - 1)
+    return {
+        "value": pairs / dt,
+        "unit": "pair interactions/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "FP64 oracle ion-ion force rows 0..%d of %d ions against all %d ions (%.1f s); "
+                  "the oracle step evaluates this O(N^2) force twice per step" % (rows, n, n, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-ions", type=int, default=2 ** 20)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pair-only", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this engine runs one simulation per GPU and this build measures one GPU only; "
+              "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    from opendust_amd import engine, workloads
+
+    N = args.n_ions
+    log("setup: single dust, N =", N)
+    t0 = time.time()
+    od = workloads.single_dust(N=N, n=args.steps + args.warmup, precision="fp32c", seed=12345,
+                               trapSeed=2024)
+    cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=7, precision="fp32c")
+    log("setup done in %.1f s (p_t=%.4f p_s=%.4f)" % (time.time() - t0, od.p_t, od.p_s))
+
+    eng = engine.Engine(cfg)
+    eng.set_ions(x0, v0)
+
+    if args.pair_only:
+        for _ in range(args.pair_only):
+            eng.pair_forces(x0)
+        log("pair-only launches done")
+        return
+
+    log("warmup", args.warmup)
+    if args.warmup:
+        eng.run(args.warmup, engine.RUN_PROFILE_PAIR)
+    eng.pair_timing(reset=True)
+    log("timed", args.steps, "steps")
+    t0 = time.perf_counter()
+    rec = eng.run(args.steps, engine.RUN_PROFILE_PAIR)  # returns after a stream synchronize
+    wall = time.perf_counter() - t0
+    pair_ms, launches = eng.pair_timing()
+    x_mid, v_mid = eng.get_ions()
+    q_mid = eng.get_dust_charges()
+    eng.close()
+
+    ms_per_step = wall / args.steps * 1e3
+    pairs_step = float(N) * (N - 1)
+    value = pairs_step * args.steps / wall
+    kernel_ms = pair_ms / max(launches, 1)
+    achieved = pairs_step * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12
+    log("fp32c: %.2f ms/step, pair kernel %.2f ms, %.3e pairs/s" % (ms_per_step, kernel_ms, value))
+
+    # FP64 GPU path and FP32c-vs-FP64 dust force error from the same state
+    fp64 = None
+    if args.fp64_steps > 0:
+        step_idx = args.warmup + args.steps
+        cfg64 = od.make_config(od.reinjection_tables(), od.alphaTrap, precision="fp64", seed=od.seed)
+        res = {}
+        for prec, c in (("fp64", cfg64), ("fp32c", cfg)):
+            e = engine.Engine(c)
+            e.set_ions(x_mid, v_mid)
+            e.set_dust_charges(q_mid)
+            e.set_step(step_idx)
+            t0 = time.perf_counter()
+            r = e.run(args.fp64_steps)
+            res[prec] = (time.perf_counter() - t0, r)
+            e.close()
+        t64, r64 = res["fp64"]
+        r32 = res["fp32c"][1]
+        fo64 = r64["force_orbit"][0, 0]
+        fo32 = r32["force_orbit"][0, 0]
+        fp64 = {
+            "steps": args.fp64_steps,
+            "ms_per_step": t64 / args.fp64_steps * 1e3,
+            "pair_interactions_per_s": pairs_step * args.fp64_steps / t64,
+            "fp32c_speedup": value / (pairs_step * args.fp64_steps / t64),
+            "dust_force_rel_err_first_step": float(np.linalg.norm(fo32 - fo64) / np.linalg.norm(fo64)),
+        }
+        log("fp64:", fp64)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(cfg, x_mid)
+        log("cpu:", cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "steps_per_s": 1e3 / ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "single dust sphere, 2^20 superions, all-pairs ion-ion force, Maxwellian Ar+ flow M=1, "
+                        "cylinder R=5 r_D_e H=10 r_D_e, consistent charge, dt=0.01/w_pi",
+            "n_ions": N,
+            "n_dust": 1,
+            "pair_arithmetic": "f32 with compensated per-ion sums",
+            "state_and_reductions": "f64",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "k_pair32<2>",
+            "kernel_ms": kernel_ms,
+            "flop_per_pair": FLOP_PER_PAIR,
+            "pairs_per_launch": pairs_step,
+        },
+        "cpu_baseline": cpu,
+        "fp64_gpu": fp64,
+        "absorbed_last_step": int(rec["absorbed"][-1, 0]),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,246 @@
+"""Host-side logic of the drop-in API (no GPU): parameter classes against the
+reference's own printed logs, reinjection tables against closed forms, the
+initial-state draw order against a literal transcription of the reference
+loop, the trap fit, restart I/O and API quirks."""
+
+import contextlib
+import io
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import special
+
+from opendust_amd import host
+from opendust_amd import opendust as od
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_logs.json")
+
+
+def _gold():
+    with open(GOLD) as fh:
+        return json.load(fh)
+
+
+def _capture(fn):
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        fn()
+    return buf.getvalue().splitlines()
+
+
+def test_maxwell_parameters_match_reference_log():
+    g = _gold()["piel2017_fig1_consistent"]
+    inp = g["inputs"]
+    pp = od.PlasmaParametersInSIUnitsMaxwell(inp["T_i"], inp["T_e"], inp["n_inf"], inp["M"], inp["m_i"])
+    sp = od.SimulationParametersInSIUnits(inp["R_over_lambda"] * pp.r_D_e, inp["H_over_lambda"] * pp.r_D_e,
+                                          inp["N"], inp["n"], inp["d_t"], pp)
+    lines = _capture(pp.printParameters) + _capture(sp.printParameters)
+    gold = g["printout"]
+    assert lines[:len(gold)] == gold and all(ln == "" for ln in lines[len(gold):])
+
+
+def test_field_driven_parameters_match_reference_log():
+    g = _gold()["patacchini2008_fig3_point1"]
+    inp = g["inputs"]
+    E = np.asarray([inp["E"]])[0]
+    w_c = np.asarray([inp["w_c"]])[0]
+    pp = od.PlasmaParametersInSIUnitsFieldDriven(inp["T_n"], inp["T_e"], inp["n_inf"], E, w_c, inp["m_i"])
+    sp = od.SimulationParametersInSIUnits(inp["R_over_lambda"] * pp.r_D_e, inp["H_over_lambda"] * pp.r_D_e,
+                                          inp["N"], inp["n"], inp["d_t"], pp)
+    lines = _capture(pp.printParameters) + _capture(sp.printParameters)
+    gold = g["printout"]
+    assert lines[:len(gold)] == gold and all(ln == "" for ln in lines[len(gold):])
+
+
+def test_alias_and_reexports():
+    import opendust_amd
+
+    assert opendust_amd.SimulatioParametersInSIUnits is opendust_amd.SimulationParametersInSIUnits
+    opendust_amd.install_alias()
+    from opendust.opendust import OpenDust  # noqa: F401  (resolves to this package)
+    from opendust import DustParticle
+
+    assert DustParticle is od.DustParticle
+
+
+def _single(N=256, n=7, dist="Maxwellian", charge="consistent"):
+    pp = od.PlasmaParametersInSIUnitsMaxwell(290.11, 29011, 1e14, 1, 6.6335209e-26)
+    lam = pp.r_D_e
+    sp = od.SimulationParametersInSIUnits(3 * lam, 6 * lam, N, n, 3.5148240854e-09, pp)
+    d = od.DustParticle(0, 0, -lam, 58.8e-6, charge, -1e5 * abs(pp.e))
+    return od.OpenDust(pp, sp, od.OutputParameters(), [d], dist)
+
+
+def test_opendust_attributes_follow_reference():
+    o = _single()
+    # t = linspace(0, n*dt, n): the reference's spacing is n/(n-1)*dt (opendust.py:315)
+    assert np.array_equal(o.t, np.linspace(0, 7 * 3.5148240854e-09, 7))
+    d = o.dustParticles[0]
+    assert d.q.shape == (7,) and np.all(d.q == d.q[0])
+    assert d.forceIonsOrbit.shape == (7, 3) and not hasattr(d, "forceExternalField")
+    assert abs(d.z / o.r_D_e + 1.0) < 1e-15
+    assert host.charge_method_code("consistent") == host.CHARGE_CONSISTENT
+    assert host.charge_method_code("oml") == host.CHARGE_OML
+    assert host.charge_method_code("anything") == host.CHARGE_GIVEN
+
+
+def test_unknown_distribution_raises_name_error():
+    with pytest.raises(NameError):
+        host.initial_velocities("uniform", 1.0, 0.1, 4)
+
+
+def _flux_cdf_closed(v_fl, v_T, a, b):
+    """int_a^b v N(v; v_fl, v_T) dv in closed form."""
+    s2 = np.sqrt(2.0) * v_T
+    return (v_T / np.sqrt(2 * np.pi) * (np.exp(-((a - v_fl) / s2) ** 2) - np.exp(-((b - v_fl) / s2) ** 2))
+            + v_fl / 2 * (special.erf((b - v_fl) / s2) - special.erf((a - v_fl) / s2)))
+
+
+@pytest.mark.parametrize("M", [0.3, 1.0, 2.0])
+def test_maxwell_face_tables_against_closed_form(M):
+    v_T = 0.2457
+    v_fl = M * 10 * v_T
+    F_top, top = host.face_table(v_fl, v_T, "top", "Maxwellian")
+    v_min, v_max = max(v_fl - 4 * v_T, 0), v_fl + 4 * v_T
+    assert abs(F_top - _flux_cdf_closed(v_fl, v_T, v_min, v_max)) <= 1e-10 * F_top
+    # knots: the CDF at every grid speed equals the closed form
+    for v, xi in zip(top.grid_v, top.grid_xi):
+        assert abs(xi - _flux_cdf_closed(v_fl, v_T, v_min, v) / F_top) <= 1e-9
+    # the expression-form evaluation equals linear interpolation of the knots
+    # up to the %.8f rounding of its constants (5e-9 each; the rounding of
+    # xi0 and the centre is amplified by the segment's slope)
+    u = np.linspace(0, 1, 2001)[:-1] + 0.00025
+    ref = np.interp(u, top.grid_xi, top.grid_v)
+    k = np.clip(np.searchsorted(top.grid_xi, u) - 1, 0, top.n_seg - 1)
+    bound = 1.5e-8 * (1.0 + np.abs(np.asarray(top.slope)[k])) + 1e-12
+    assert np.all(np.abs(top(u) - ref) <= bound)
+    F_side, side = host.face_table(0, v_T, "side", "Maxwellian")
+    assert abs(F_side - _flux_cdf_closed(0, v_T, 0, 4 * v_T)) <= 1e-10 * F_side
+    F_bot, bot = host.face_table(-v_fl, v_T, "bottom", "Maxwellian")
+    if -v_fl + 4 * v_T < 0.4 * v_T:
+        assert F_bot == 0 and bot is None
+    else:
+        assert np.all(bot.grid_v <= 0)
+
+
+def test_face_probabilities_single_dust_workload():
+    from opendust_amd import workloads
+
+    o = workloads.single_dust(N=1024, n=2)
+    o.reinjection_tables()
+    assert abs(o.p_t - 0.8624) < 1e-4 and abs(o.p_s - 0.1376) < 1e-4 and o.p_b == 0.0
+    top = host.face_table(o.v_fl, o.v_T, "top", "Maxwellian")[1]
+    assert top.n_seg == 38  # int(8 v_T / (0.2 v_T)) = 39 knots after float truncation
+
+
+def test_field_driven_tables_monotone_and_normalised():
+    v_T, v_fl = 0.2457, 8 * 0.2457
+    for side, vf in (("top", v_fl), ("bottom", -v_fl), ("side", 0)):
+        F, t = host.face_table(vf, v_T, side, "fieldDriven")
+        assert F > 0
+        assert t.grid_xi[0] == 0.0 and abs(t.grid_xi[-1] - 1.0) < 1e-12
+        assert np.all(np.diff(t.grid_xi) > 0)
+    vz = host.vz_table(v_fl, v_T)
+    assert np.all(np.diff(vz.grid_xi) > 0) and abs(vz.grid_xi[-1] - 1) < 1e-12
+
+
+def test_u32_threshold():
+    assert host.u32_threshold(0.0) == 0
+    assert host.u32_threshold(1.0) == 2 ** 32
+    assert host.u32_threshold(0.5) == 2 ** 31
+    assert host.u32_threshold(1e-3) == int(np.floor(1e-3 * 2 ** 32))
+
+
+def _reference_loop_positions(R, H, dusts, N):
+    """Literal transcription of the reference's per-ion rejection loop
+    (opendust.py:376-399) for comparison with the vectorised version."""
+    from numpy import random
+
+    position = []
+    for _ in range(N):
+        flag = True
+        while flag:
+            r = np.sqrt(random.uniform(0, 1))
+            theta = random.uniform(0, 1)
+            x = R * r * np.cos(2 * np.pi * theta)
+            y = R * r * np.sin(2 * np.pi * theta)
+            z = random.uniform(-H / 2, H / 2)
+            inside = False
+            for (dx, dy, dz, dr) in dusts:
+                if np.sqrt((x - dx) ** 2 + (y - dy) ** 2 + (z - dz) ** 2) <= dr:
+                    inside = True
+                    break
+            if not inside:
+                position.append((x, y, z))
+                flag = False
+    return np.array(position)
+
+
+def test_initial_positions_follow_reference_stream():
+    R, H = 3.0, 6.0
+    dusts = [(0.0, 0.0, -1.0, 1.2), (0.5, 0.0, 1.0, 0.8)]  # big dusts: many rejections
+    np.random.seed(123)
+    ref = _reference_loop_positions(R, H, dusts, 400)
+    nxt_ref = np.random.random_sample()
+    np.random.seed(123)
+    got = host.initial_positions(R, H, dusts, 400)
+    nxt_got = np.random.random_sample()
+    assert np.max(np.abs(got - ref)) <= 4 * np.finfo(float).eps * R
+    assert nxt_got == nxt_ref  # same stream position afterwards
+
+
+def test_initial_velocities_same_calls():
+    np.random.seed(9)
+    v = host.initial_velocities("Maxwellian", 2.0, 0.25, 100)
+    np.random.seed(9)
+    ref = np.stack([np.random.normal(0, 0.25, 100), np.random.normal(0, 0.25, 100),
+                    np.random.normal(2.0, 0.25, 100)], axis=1)
+    assert np.array_equal(v, ref)
+
+
+def test_trap_fit_recovers_polynomial():
+    Rl, Hl = 5.0, 10.0
+    n_r, n_h, n_mc = host.trap_grid_shape(Rl, Hl)
+    assert (n_r, n_h) == (50, 50) and n_mc == int(np.pi * 25 * 10 / 0.05 ** 3)
+    rng = np.random.RandomState(0)
+    c = rng.normal(size=15) * 10.0 ** -np.arange(15)
+    rho = np.linspace(0, Rl, n_r)
+    hh = np.linspace(0, Hl / 2, n_h)
+    R2, H2 = np.meshgrid(rho, hh, indexing="ij")
+    pot = sum(c[m] * R2 ** a * H2 ** b for m, (a, b) in enumerate(host.TRAP_MONOMIALS))
+    alpha = host.trap_fit(pot, Rl, Hl, 1000)
+    assert np.allclose(alpha, -c * 1000, rtol=1e-6, atol=1e-9 * np.abs(c).max() * 1000)
+
+
+def test_restart_roundtrip_and_errors(tmp_path):
+    o = _single(N=5)
+    pos = np.arange(15, dtype=float).reshape(5, 3) * 1.5
+    vel = -np.arange(15, dtype=float).reshape(5, 3) * 0.25
+    o.outputParameters.restartFileName = str(tmp_path / "RESTART")
+    o._write_restart(pos, vel)
+    x, v = host.read_restart(str(tmp_path / "RESTART"), 5)
+    assert np.array_equal(x, pos) and np.array_equal(v, vel)
+    bad = tmp_path / "BAD"
+    bad.write_text("1\t2\t3\n")
+    with pytest.raises(ValueError):
+        host.read_restart(str(bad), 1)
+
+
+def test_config_layout_and_constants():
+    from opendust_amd import workloads
+
+    o = workloads.single_dust(N=512, n=3)
+    alpha = np.zeros(15)
+    cfg, x0, v0 = workloads.prepare(o, alpha=alpha, np_seed=1)
+    assert cfg.n_ions == 512 and cfg.n_dust == 1
+    # v_B dt = 0.01 r_D_e at dt = 0.01/w_pi, M = 1 (v_B / w_pi = r_D_e)
+    assert abs(cfg.v_fl * cfg.dt / cfg.r_D_e - 0.01) < 1e-9
+    assert abs(cfg.v_fl / cfg.v_T - 10.0) < 1e-9
+    assert cfg.top.n_seg == 38 and cfg.side.n_seg > 0 and cfg.bottom.n_seg == 0
+    assert cfg.use_bottom == 0
+    # positions inside the cylinder, outside the dust
+    rho = np.hypot(x0[:, 0], x0[:, 1])
+    assert np.all(rho <= cfg.R) and np.all(np.abs(x0[:, 2]) <= cfg.H / 2)
+    assert np.all(np.linalg.norm(x0, axis=1) > cfg.dust_r[0])
