@@ -56,11 +56,14 @@ def cpu_baseline(cfg, x, target_s=12.0):
 
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.threads()
     n = x.shape[0]
-    rows = 64
-    t0 = time.perf_counter()
-    oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
-    t1 = time.perf_counter() - t0
-    rows = int(min(n, max(64, rows * target_s / max(t1, 1e-3))))
+    rows = 8 * nthreads
+    t1 = 0.0
+    while t1 < 1.0 and rows < n:
+        rows = min(n, rows * 4)
+        t0 = time.perf_counter()
+        oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
+        t1 = time.perf_counter() - t0
+    rows = int(min(n, max(rows, rows * target_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
     dt = time.perf_counter() - t0
