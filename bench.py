@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pair-only", type=int, default=0)
+    ap.add_argument("--pair-mode", choices=["sym", "rows", "auto"], default="sym")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,10 +107,12 @@ def main():
     od = workloads.single_dust(N=N, n=args.steps + args.warmup, precision="fp32c", seed=12345,
                                trapSeed=2024)
     cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=7, precision="fp32c")
+    cfg.pair_mode = {"rows": engine.PAIR_ROWS, "sym": engine.PAIR_SYM, "auto": engine.PAIR_AUTO}[args.pair_mode]
     log("setup done in %.1f s (p_t=%.4f p_s=%.4f)" % (time.time() - t0, od.p_t, od.p_s))
 
     eng = engine.Engine(cfg)
     eng.set_ions(x0, v0)
+    mode, pairs_launch = eng.pair_info()
 
     if args.pair_only:
         for _ in range(args.pair_only):
@@ -134,7 +137,7 @@ def main():
     pairs_step = float(N) * (N - 1)
     value = pairs_step * args.steps / wall
     kernel_ms = pair_ms / max(launches, 1)
-    achieved = pairs_step * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12
+    achieved = pairs_launch * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12
     log("fp32c: %.2f ms/step, pair kernel %.2f ms, %.3e pairs/s" % (ms_per_step, kernel_ms, value))
 
     # FP64 GPU path and FP32c-vs-FP64 dust force error from the same state
@@ -200,10 +203,10 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "k_pair32<2>",
+            "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
             "flop_per_pair": FLOP_PER_PAIR,
-            "pairs_per_launch": pairs_step,
+            "pairs_per_launch": pairs_launch,
         },
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,

@@ -45,6 +45,11 @@ extern "C" {
 #define OD_PREC_FP32C 0 /* FP32 pair arithmetic, compensated (Kahan) per-ion sums */
 #define OD_PREC_FP64 1  /* plain FP64 pair arithmetic */
 
+/* all-pairs kernel (FP32 path; the FP64 path always uses the row kernel) */
+#define OD_PAIR_AUTO 0 /* symmetric when the problem is large enough, else rows */
+#define OD_PAIR_ROWS 1 /* every ordered pair, one target row per thread */
+#define OD_PAIR_SYM 2  /* every unordered pair once (Newton's third law) */
+
 /* charge methods (opendust.py:1120, :1153; any other string keeps q) */
 #define OD_CHARGE_GIVEN 0
 #define OD_CHARGE_CONSISTENT 1
@@ -74,7 +79,9 @@ typedef struct od_config {
   int32_t precision;     /* OD_PREC_* */
   int32_t device;        /* HIP device ordinal */
   int32_t use_bottom;    /* reference applies the bottom-face selects only if p_b != 0 (opendust.py:939) */
-  int32_t j_split;       /* pair kernel j-chunks; 0 = automatic */
+  int32_t j_split;       /* source chunks of the row pair kernel; 0 = automatic */
+  int32_t pair_mode;     /* OD_PAIR_*: which all-pairs kernel */
+  int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   uint64_t seed;         /* Philox4x32-10 key */
 
   /* geometry, integration (OpenMM units) */
@@ -164,6 +171,12 @@ int od_run(od_ctx* ctx, int64_t nsteps, od_dust_record* out, int32_t flags);
 /* Accumulated HIP-event time of the main pair launches timed with
  * OD_RUN_PROFILE_PAIR since the last reset, and their count. */
 int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
+
+/* Which all-pairs kernel the context runs (OD_PAIR_ROWS or OD_PAIR_SYM) and
+ * how many ordered ion pairs its main launch (the one OD_RUN_PROFILE_PAIR
+ * times) covers: N(N-1) for rows, N(N-1) minus the intra-block pairs for the
+ * symmetric kernel.  No counterpart in the reference (measurement support). */
+int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
 /* Diagnostics of the last step (for parity tests): per-ion flag word
  * (bit p < 16: inside dust p; bit 16: outside in r; bit 17: outside in z),

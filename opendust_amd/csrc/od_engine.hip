@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "od_device.h"
@@ -67,6 +68,7 @@ static int set_err(int code, const char* fmt, ...) {
 struct DevParams {
   ExtParams ext;
   int n, n_pad, n_dust, distribution, use_bottom;
+  uint32_t dust_mask;  // flag bits of the dusts
   uint64_t seed;
   double R, H, half_H, dt, half_dt, m_s, v_T, v_fl, lam;
   double pos_scale;   // nm -> kernel units (FP32: L/lambda, FP64: 1/lambda)
@@ -260,6 +262,215 @@ __global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Symmetric all-pairs ion-ion force, FP32 (every unordered pair once).
+//
+// Ions are cut into blocks of SYM_TB.  A workgroup owns one target block I
+// and `ch` consecutive source blocks J > I (upper triangle); pairs inside a
+// block are done by k_pair32_diag.  Inside a workgroup each wave keeps 4
+// target sub-blocks of 64 ions in registers and streams 64-ion source
+// sub-blocks.  Within a 16-lane row, step s pairs lane c's targets with the
+// source of lane (c - s) mod 16: the source position is read through a DPP
+// row-rotate folded into the subtraction, and the source's reaction
+// accumulator is rotated by one lane per step, so after 16 steps it is home.
+// Rows exchange sources (xor 16 / xor 32) between the four phases, which
+// covers all 64 x 64 pairs.  The four waves' reactions on a source sub-block
+// are summed in a fixed order through LDS and stored as one partial per
+// (workgroup, source); target sums are Kahan-compensated across sub-blocks.
+// k_sym_reduce adds diagonal, target and source partials in a fixed order.
+// 13 + 3 + 3 VALU per unordered pair plus one 3-register rotate per 4 pairs.
+// ---------------------------------------------------------------------------
+constexpr int SYM_TB = 1024;  // ions per block of the triangular schedule
+constexpr int SYM_MT = 4;     // 64-target sub-blocks per wave
+
+template <int S>
+__device__ __forceinline__ float dpp_ror(float v) {
+  if constexpr (S == 0) {
+    return v;
+  } else {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
+                                                                 __builtin_bit_cast(int, v), 0x120 + S, 0xF,
+                                                                 0xF, true));
+  }
+}
+
+struct SymTargets {
+  float x[SYM_MT], y[SYM_MT], z[SYM_MT];
+  float ax[SYM_MT], ay[SYM_MT], az[SYM_MT];
+};
+
+// Empty asm with a read-write register operand: the value must exist at this
+// point, so the optimiser can neither hoist a step's source reads above it nor
+// sink an accumulator chain below it (either blows up register pressure).
+__device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
+
+template <int S>
+__device__ __forceinline__ void sym_step(SymTargets& T, float xj, float yj, float zj, float& bx, float& by,
+                                         float& bz, float a, float linv) {
+  const float xs = dpp_ror<S>(xj), ys = dpp_ror<S>(yj), zs = dpp_ror<S>(zj);
+#pragma unroll
+  for (int k = 0; k < SYM_MT; ++k) {
+    const float dx = xs - T.x[k];
+    const float dy = ys - T.y[k];
+    const float dz = zs - T.z[k];
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32)));
+    const float r = __builtin_amdgcn_sqrtf(r2);
+    const float e = __builtin_amdgcn_exp2f(-r);
+    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(a, r, r2));
+    const float t = __builtin_fmaf(r, w, linv);
+    const float h = e * w * t;
+    T.ax[k] = __builtin_fmaf(h, dx, T.ax[k]);
+    T.ay[k] = __builtin_fmaf(h, dy, T.ay[k]);
+    T.az[k] = __builtin_fmaf(h, dz, T.az[k]);
+    bx = __builtin_fmaf(-h, dx, bx);
+    by = __builtin_fmaf(-h, dy, by);
+    bz = __builtin_fmaf(-h, dz, bz);
+  }
+  bx = dpp_ror<1>(bx);
+  by = dpp_ror<1>(by);
+  bz = dpp_ror<1>(bz);
+  pin(bx);
+  pin(by);
+  pin(bz);
+#pragma unroll
+  for (int k = 0; k < SYM_MT; ++k) {
+    pin(T.ax[k]);
+    pin(T.ay[k]);
+    pin(T.az[k]);
+  }
+}
+
+template <int... S>
+__device__ __forceinline__ void sym_phase(std::integer_sequence<int, S...>, SymTargets& T, float xj, float yj,
+                                          float zj, float& bx, float& by, float& bz, float a, float linv) {
+  (sym_step<S>(T, xj, yj, zj, bx, by, bz, a, linv), ...);
+}
+
+__device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& zj, float& bx, float& by,
+                                         float& bz) {
+  xj = __shfl_xor(xj, mask, 64);
+  yj = __shfl_xor(yj, mask, 64);
+  zj = __shfl_xor(zj, mask, 64);
+  bx = __shfl_xor(bx, mask, 64);
+  by = __shfl_xor(by, mask, 64);
+  bz = __shfl_xor(bz, mask, 64);
+}
+
+// tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials
+__global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
+                                                    const int2* __restrict__ wg_tab, float a, float linv,
+                                                    double* __restrict__ tp, float* __restrict__ sp) {
+  __shared__ double red[4][3][64];
+  const int2 ik = wg_tab[blockIdx.x];
+  const int I = ik.x;
+  const int J0 = I + 1 + ik.y * ch;
+  const int J1 = min(J0 + ch, n_blocks);
+  const int nsub = (J1 - J0) * (SYM_TB / 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t sp_len = (size_t)ch * SYM_TB;
+  SymTargets T;
+  float sx[SYM_MT], sy[SYM_MT], sz[SYM_MT], cx[SYM_MT], cy[SYM_MT], cz[SYM_MT];
+#pragma unroll
+  for (int k = 0; k < SYM_MT; ++k) {
+    const float4 p = pos[(size_t)I * SYM_TB + wave * (64 * SYM_MT) + k * 64 + lane];
+    T.x[k] = p.x;
+    T.y[k] = p.y;
+    T.z[k] = p.z;
+    sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
+  }
+  const float4* src = pos + (size_t)J0 * SYM_TB + lane;
+  float* spw = sp + (size_t)blockIdx.x * 3 * sp_len;
+  const auto seq = std::make_integer_sequence<int, 16>{};
+  for (int b = 0; b < nsub; ++b) {
+    const float4 p = src[(size_t)b * 64];
+    float xj = p.x, yj = p.y, zj = p.z;
+    float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+    for (int k = 0; k < SYM_MT; ++k) T.ax[k] = T.ay[k] = T.az[k] = 0.f;
+    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
+    sym_xchg(16, xj, yj, zj, bx, by, bz);
+    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
+    sym_xchg(32, xj, yj, zj, bx, by, bz);
+    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
+    sym_xchg(16, xj, yj, zj, bx, by, bz);
+    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
+    bx = __shfl_xor(bx, 32, 64);
+    by = __shfl_xor(by, 32, 64);
+    bz = __shfl_xor(bz, 32, 64);
+#pragma unroll
+    for (int k = 0; k < SYM_MT; ++k) {
+      kahan_add(sx[k], cx[k], T.ax[k]);
+      kahan_add(sy[k], cy[k], T.ay[k]);
+      kahan_add(sz[k], cz[k], T.az[k]);
+    }
+    red[wave][0][lane] = bx;
+    red[wave][1][lane] = by;
+    red[wave][2][lane] = bz;
+    __syncthreads();
+    if (threadIdx.x < 192) {
+      const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+      const double v = ((red[0][c][l] + red[1][c][l]) + red[2][c][l]) + red[3][c][l];
+      spw[c * sp_len + (size_t)b * 64 + l] = (float)v;
+    }
+    __syncthreads();
+  }
+  double* tpw = tp + (size_t)blockIdx.x * 3 * SYM_TB;
+#pragma unroll
+  for (int k = 0; k < SYM_MT; ++k) {
+    const int t = wave * (64 * SYM_MT) + k * 64 + lane;
+    tpw[t] = (double)sx[k] - (double)cx[k];
+    tpw[SYM_TB + t] = (double)sy[k] - (double)cy[k];
+    tpw[2 * SYM_TB + t] = (double)sz[k] - (double)cz[k];
+  }
+}
+
+// pairs inside one block (ordered, sources via scalar loads); dg: [3][n_pad]
+__global__ __launch_bounds__(256) void k_pair32_diag(const float4* __restrict__ pos, int n_pad, float a,
+                                                     float linv, double* __restrict__ dg) {
+  const int X = blockIdx.x / (SYM_TB / 256), q = blockIdx.x % (SYM_TB / 256);
+  const int i = X * SYM_TB + q * 256 + threadIdx.x;
+  const float4 pi = pos[i];
+  float ax = 0.f, ay = 0.f, az = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  const float4* src = pos + (size_t)X * SYM_TB;
+  for (int jt = 0; jt < SYM_TB; jt += TJ) {
+    ax = ay = az = 0.f;
+#pragma unroll 8
+    for (int jj = 0; jj < TJ; ++jj) {
+      const float4 p = src[jt + jj];
+      pair32(pi.x, pi.y, pi.z, p.x, p.y, p.z, a, linv, ax, ay, az);
+    }
+    kahan_add(sx, cx, ax);
+    kahan_add(sy, cy, ay);
+    kahan_add(sz, cz, az);
+  }
+  dg[i] = (double)sx - (double)cx;
+  dg[n_pad + i] = (double)sy - (double)cy;
+  dg[2 * n_pad + i] = (double)sz - (double)cz;
+}
+
+// F = scale * (diagonal + target partials + source partials), fixed order
+__global__ __launch_bounds__(256) void k_sym_reduce(const double* __restrict__ dg, const double* __restrict__ tp,
+                                                    const float* __restrict__ sp, const int* __restrict__ wg_base,
+                                                    int n_blocks, int ch, int n_pad, double scale,
+                                                    double* __restrict__ F) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_pad) return;
+  const int X = i / SYM_TB, off = i % SYM_TB;
+  const size_t sp_len = (size_t)ch * SYM_TB;
+  const int k0 = wg_base[X], k1 = wg_base[X + 1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = dg[(size_t)c * n_pad + i];
+    for (int J = 0; J < X; ++J) {
+      const int d = X - J - 1;
+      const size_t wg = (size_t)wg_base[J] + d / ch;
+      s += (double)sp[(wg * 3 + c) * sp_len + (size_t)(d % ch) * SYM_TB + off];
+    }
+    for (int w = k0; w < k1; ++w) s += tp[((size_t)w * 3 + c) * SYM_TB + off];
+    F[(size_t)c * n_pad + i] = s * scale;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // step kernels
 // ---------------------------------------------------------------------------
 constexpr uint32_t FLAG_OUT_R = 1u << 16;
@@ -443,7 +654,7 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
                                                const int* __restrict__ slot_of,
                                                const typename Pos<T>::V* __restrict__ list_old,
                                                typename Pos<T>::V* __restrict__ list_new, uint8_t* __restrict__ face_out,
-                                               double* __restrict__ partials) {
+                                               const double* __restrict__ rowD, double* __restrict__ partials) {
   __shared__ double lds[4];
   const int n = P->n, np = P->n_pad;
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -470,9 +681,18 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
     vz = V[2 * np + i];
     double fx, fy, fz;
     ext_force(P->ext, q, x, y, z, fx, fy, fz);
-    fx += F[i] + ax.total() * P->pair_scale;
-    fy += F[np + i] + ay.total() * P->pair_scale;
-    fz += F[2 * np + i] + az.total() * P->pair_scale;
+    if (fl & P->dust_mask) {
+      // absorbed: its full row at the drifted state (the main kernel parks flagged ions)
+      const int s = slot_of[i];
+      fx += rowD[3 * s];
+      fy += rowD[3 * s + 1];
+      fz += rowD[3 * s + 2];
+    } else {
+      // flagged ions that only left the domain are reinjected below, whatever this kick gives
+      fx += F[i] + ax.total() * P->pair_scale;
+      fy += F[np + i] + ay.total() * P->pair_scale;
+      fz += F[2 * np + i] + az.total() * P->pair_scale;
+    }
     vx = vx + P->half_dt * fx / P->m_s;
     vy = vy + P->half_dt * fy / P->m_s;
     vz = vz + P->half_dt * fz / P->m_s;
@@ -516,16 +736,20 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
   }
 }
 
-// Full force rows of the reinjected ions at the post-reinjection state.
+// Full force rows of listed ions against all sources `pj`: the reinjected ions
+// at the post-reinjection state (need_mask = 0), or the absorbed ions at the
+// drifted state (need_mask = dust bits: other listed ions are skipped).
 template <class T>
 __global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ P,
                                               const typename Pos<T>::V* __restrict__ pj, const int* __restrict__ n_flag,
                                               const typename Pos<T>::V* __restrict__ list_new,
-                                              double* __restrict__ rowF) {
+                                              const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
+                                              uint32_t need_mask, double* __restrict__ rowF) {
   __shared__ double lds[4];
   const int nf = n_flag[0];
   const int np = P->n_pad;
   for (int s = blockIdx.x; s < nf; s += gridDim.x) {
+    if (need_mask != 0u && (flags[list_ion[s]] & need_mask) == 0u) continue;
     const typename Pos<T>::V own = list_new[s];
     Acc<T> ax, ay, az;
     for (int j0 = threadIdx.x; j0 < np; j0 += 256 * TJ) {
@@ -764,6 +988,12 @@ struct od_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double pair_ms = 0.0;
   int64_t pair_launches = 0;
+  // symmetric pair kernel
+  int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0;
+  int2* wg_tab = nullptr;
+  int* wg_base = nullptr;
+  double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
+  float* sym_sp = nullptr;
 };
 
 static int fill_params(const od_config* c, int n_pad, DevParams* P) {
@@ -774,6 +1004,7 @@ static int fill_params(const od_config* c, int n_pad, DevParams* P) {
   P->n_dust = c->n_dust;
   P->distribution = c->distribution;
   P->use_bottom = c->use_bottom;
+  P->dust_mask = (c->n_dust > 0) ? ((1u << c->n_dust) - 1u) : 0u;
   P->seed = c->seed;
   P->R = c->R;
   P->H = c->H;
@@ -847,7 +1078,8 @@ static void free_ctx(od_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
-                  c->slot_of, c->list_ion, c->err, c->rec};
+                  c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
+                  c->rowD, c->sym_sp};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -877,6 +1109,51 @@ static void pair_geometry(int n_pad, int ib, int j_split, int* n_chunks, int* ch
   len = (len + TJ - 1) / TJ * TJ;
   *n_chunks = c;
   *chunk_len = len;
+}
+
+// workgroups of the triangular schedule: target block I, source blocks
+// I+1+k*ch .. I+(k+1)*ch (clipped), k = 0, 1, ...
+static int64_t sym_wg_count(int n_blocks, int ch) {
+  int64_t n = 0;
+  for (int I = 0; I < n_blocks; ++I) n += (n_blocks - 1 - I + ch - 1) / ch;
+  return n;
+}
+
+static int setup_sym(od_ctx* c, const od_config* cfg) {
+  c->sym = 0;
+  if (c->prec != OD_PREC_FP32C || cfg->pair_mode == OD_PAIR_ROWS) return OD_OK;
+  const int T = c->n_pad / SYM_TB;
+  const bool forced = cfg->pair_mode == OD_PAIR_SYM;
+  if (T < 2 || (!forced && c->n_pad < (1 << 17))) return OD_OK;
+  int ch = cfg->sym_chunk > 0 ? cfg->sym_chunk : 16;
+  if (cfg->sym_chunk <= 0)
+    while (ch > 1 && sym_wg_count(T, ch) < 4096) ch /= 2;
+  const int64_t n_wg = sym_wg_count(T, ch);
+  const double bytes = (double)n_wg * 3 * SYM_TB * (4.0 * ch + 8.0);
+  if (!forced && bytes > 48e9) return OD_OK;
+  if (n_wg > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "symmetric pair kernel: too many workgroups");
+  std::vector<int2> tab;
+  std::vector<int> base(T + 1);
+  tab.reserve(n_wg);
+  for (int I = 0; I < T; ++I) {
+    base[I] = (int)tab.size();
+    const int nk = (T - 1 - I + ch - 1) / ch;
+    for (int k = 0; k < nk; ++k) tab.push_back(make_int2(I, k));
+  }
+  base[T] = (int)tab.size();
+  int rc = OD_OK;
+  if ((rc = dalloc(&c->wg_tab, tab.size() * sizeof(int2))) || (rc = dalloc(&c->wg_base, base.size() * sizeof(int))) ||
+      (rc = dalloc(&c->sym_tp, (size_t)n_wg * 3 * SYM_TB * 8)) ||
+      (rc = dalloc(&c->sym_sp, (size_t)n_wg * 3 * ch * SYM_TB * 4)) ||
+      (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)))
+    return rc;
+  HIPCHK(hipMemcpy(c->wg_tab, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->wg_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->sym = 1;
+  c->n_blocks = T;
+  c->sym_ch = ch;
+  c->n_wg = (int)n_wg;
+  return OD_OK;
 }
 
 static int check_config(const od_config* c) {
@@ -941,7 +1218,8 @@ extern "C" int od_create(const od_config* cfg, od_ctx** out) {
       (rc = dalloc(&c->st, 4 * 4)) || (rc = dalloc(&c->face, np)) || (rc = dalloc(&c->block_cnt, c->nb * 4)) ||
       (rc = dalloc(&c->block_off, c->nb * 4)) || (rc = dalloc(&c->n_flag, 4)) ||
       (rc = dalloc(&c->slot_of, np * 4)) || (rc = dalloc(&c->list_ion, np * 4)) || (rc = dalloc(&c->err, 4)) ||
-      (rc = dalloc(&c->rec, (size_t)c->rec_cap * std::max(1, cfg->n_dust) * sizeof(od_dust_record))))
+      (rc = dalloc(&c->rec, (size_t)c->rec_cap * std::max(1, cfg->n_dust) * sizeof(od_dust_record))) ||
+      (rc = dalloc(&c->rowD, 3 * np * 8)) || (rc = setup_sym(c, cfg)))
     return fail(rc);
   DevParams hp;
   fill_params(cfg, c->n_pad, &hp);
@@ -965,15 +1243,27 @@ extern "C" int od_destroy(od_ctx* c) {
   return OD_OK;
 }
 
-// pair force of the current (pi, pj) into F, optionally timed
+// Pair force of sources pj on targets pi into F, optionally timed.  The
+// symmetric kernel treats one array as both targets and sources: it is only
+// used when pi == pj.
 static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, bool timed) {
-  const DevParams* dP = c->dP;
-  (void)dP;
+  const double L = 1.4426950408889634;
+  const double C = c->cfg.kqq / (c->cfg.r_D_e * c->cfg.r_D_e);
+  if (c->sym && pi == pj) {
+    const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
+    if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(256), 0, c->stream, (const float4*)pj, c->n_blocks,
+                       c->sym_ch, (const int2*)c->wg_tab, a, linv, c->sym_tp, c->sym_sp);
+    if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+    hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
+                       (const float4*)pj, c->n_pad, a, linv, c->sym_dg);
+    hipLaunchKernelGGL(k_sym_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->sym_dg, c->sym_tp,
+                       c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, F);
+    return OD_OK;
+  }
   double* out = (c->n_chunks > 1) ? c->Fpart : F;
   const int n_iblk = c->n_pad / (256 * c->ib);
   const int grid = n_iblk * c->n_chunks;
-  const double L = 1.4426950408889634;
-  const double C = c->cfg.kqq / (c->cfg.r_D_e * c->cfg.r_D_e);
   if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (c->prec == OD_PREC_FP32C) {
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
@@ -1001,12 +1291,17 @@ static int launch_step_t(od_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->block_cnt, nb, c->block_off, c->n_flag);
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, c->stream, c->flags, c->block_cnt, c->block_off, c->n,
                      (const PV*)c->pi, c->slot_of, c->list_ion, (PV*)c->list_old);
-  CHKRET(launch_pair(c, c->pi, c->pj, c->F, timed));
+  // flagged ions are parked in pj, as sources and as targets
+  CHKRET(launch_pair(c, c->pj, c->pj, c->F, timed));
+  const uint32_t dust_bits = (c->cfg.n_dust > 0) ? ((1u << c->cfg.n_dust) - 1u) : 0u;
+  if (dust_bits)
+    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pi, c->n_flag,
+                       (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
                      (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
-                     (PV*)c->list_new, c->face, c->partials);
+                     (PV*)c->list_new, c->face, c->rowD, c->partials);
   hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pj, c->n_flag,
-                     (const PV*)c->list_new, c->rowF);
+                     (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->F, (const PV*)c->pj, c->flags,
                      c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
   hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, nb, c->q, c->st, c->rec);
@@ -1048,7 +1343,7 @@ extern "C" int od_set_ions(od_ctx* c, const double* x, const double* v, int64_t 
   HIPCHK(hipMemcpyAsync(c->X, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->V, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, c->stream));
   CHKRET(load_positions(c, c->X, c->pi, c->pj));
-  CHKRET(launch_pair(c, c->pi, c->pj, c->F, false));
+  CHKRET(launch_pair(c, c->pj, c->pj, c->F, false));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return OD_OK;
@@ -1174,6 +1469,26 @@ extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t fl
   return OD_OK;
 }
 
+extern "C" int od_pair_info(od_ctx* c, int32_t* mode, int64_t* pairs_per_launch) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  const int64_t n = c->n;
+  if (mode) *mode = c->sym ? OD_PAIR_SYM : OD_PAIR_ROWS;
+  if (pairs_per_launch) {
+    if (c->sym) {
+      // ordered pairs between different blocks (the main kernel's share)
+      int64_t diag = 0;
+      for (int X = 0; X < c->n_blocks; ++X) {
+        const int64_t nx = std::max<int64_t>(0, std::min<int64_t>(SYM_TB, n - (int64_t)X * SYM_TB));
+        diag += nx * (nx - 1);
+      }
+      *pairs_per_launch = n * (n - 1) - diag;
+    } else {
+      *pairs_per_launch = n * (n - 1);
+    }
+  }
+  return OD_OK;
+}
+
 extern "C" int od_pair_timing(od_ctx* c, double* total_ms, int64_t* launches, int32_t reset) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   if (total_ms) *total_ms = c->pair_ms;
@@ -1218,7 +1533,7 @@ extern "C" int od_pair_forces(od_ctx* c, const double* x, double* f) {
   hipError_t e = hipMemcpyAsync(dX, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     rc = load_positions(c, dX, pi, pj);
-    if (rc == OD_OK) rc = launch_pair(c, pi, pj, dF, false);
+    if (rc == OD_OK) rc = launch_pair(c, pj, pj, dF, false);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(sf.data(), dF, sf.size() * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

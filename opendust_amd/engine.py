@@ -24,6 +24,10 @@ ABI_VERSION = 1
 RUN_GRAPH = 1
 RUN_PROFILE_PAIR = 2
 
+PAIR_AUTO = 0
+PAIR_ROWS = 1
+PAIR_SYM = 2
+
 _dv = ctypes.c_double * MAX_DUST
 
 
@@ -61,6 +65,8 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("use_bottom", ctypes.c_int32),
         ("j_split", ctypes.c_int32),
+        ("pair_mode", ctypes.c_int32),
+        ("sym_chunk", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("R", ctypes.c_double), ("H", ctypes.c_double), ("dt", ctypes.c_double),
         ("r_D_e", ctypes.c_double), ("r_0", ctypes.c_double),
@@ -126,7 +132,7 @@ _lib = None
 # every symbol the header declares
 EXPORTS = [
     "od_create", "od_set_ions", "od_get_ions", "od_set_dust_charges", "od_get_dust_charges",
-    "od_set_step", "od_run", "od_pair_timing", "od_get_step_info", "od_pair_forces",
+    "od_set_step", "od_run", "od_pair_timing", "od_pair_info", "od_get_step_info", "od_pair_forces",
     "od_trap_potential_grid", "od_trap_samples_host", "od_abi_sizes", "od_destroy",
     "od_last_error", "od_build_info",
 ]
@@ -153,6 +159,7 @@ def load_library(path=LIB_PATH):
     lib.od_set_step.argtypes = [c_ctx, ctypes.c_int64]
     lib.od_run.argtypes = [c_ctx, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32]
     lib.od_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), ctypes.c_int32]
+    lib.od_pair_info.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int64)]
     lib.od_get_step_info.argtypes = [c_ctx, P(ctypes.c_uint32), P(ctypes.c_uint8), P(ctypes.c_int64)]
     lib.od_pair_forces.argtypes = [c_ctx, dp, dp]
     lib.od_trap_potential_grid.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, dp,
@@ -266,6 +273,13 @@ class Engine:
         _check(self.lib.od_pair_timing(self._ctx, ctypes.byref(ms), ctypes.byref(n), int(bool(reset))),
                "od_pair_timing")
         return ms.value, n.value
+
+    def pair_info(self):
+        """(mode, ordered pairs covered by one timed main pair launch)"""
+        mode = ctypes.c_int32()
+        pairs = ctypes.c_int64()
+        _check(self.lib.od_pair_info(self._ctx, ctypes.byref(mode), ctypes.byref(pairs)), "od_pair_info")
+        return mode.value, pairs.value
 
     def step_info(self):
         flags = np.empty(self.n, dtype=np.uint32)

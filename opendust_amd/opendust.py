@@ -295,6 +295,8 @@ class OpenDust:
         c.device = int(device)
         c.use_bottom = 1 if self.p_b != 0 else 0
         c.j_split = 0
+        c.pair_mode = 0
+        c.sym_chunk = 0
         c.seed = int(seed) & (2 ** 64 - 1)
         c.R, c.H, c.dt = self.R, self.H, self.d_t
         c.r_D_e, c.r_0 = self.r_D_e, self.r_0

@@ -413,3 +413,6 @@ int orc_max_threads(void) {
   return 1;
 #endif
 }
+
+/* layout check against the ctypes mirror */
+int64_t orc_config_size(void) { return (int64_t)sizeof(od_config); }

@@ -27,8 +27,11 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        build()
+    # make rebuilds only when od_oracle.c or include/opendust_amd.h changed;
+    # a stale library would read od_config with the wrong layout
+    if os.path.exists(os.path.join(HERE, "od_oracle.c")) and subprocess.run(
+            ["make", "-s", "-C", HERE], capture_output=True).returncode != 0 and not os.path.exists(LIB_PATH):
+        raise RuntimeError("building the oracle failed")
     from opendust_amd.engine import Config, DustRecord, Table  # structure layouts only
 
     lib = ctypes.CDLL(LIB_PATH)
@@ -56,6 +59,9 @@ def load():
     lib.orc_trap_grid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_int, dp,
                                   ctypes.c_int64, ctypes.c_uint64, dp, ctypes.c_int]
     lib.orc_max_threads.restype = ctypes.c_int
+    lib.orc_config_size.restype = ctypes.c_int64
+    if lib.orc_config_size() != ctypes.sizeof(Config):
+        raise RuntimeError("oracle library was built against a different od_config layout; run make -C oracle")
     _lib = lib
     return lib
 

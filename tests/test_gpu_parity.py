@@ -32,12 +32,16 @@ def _engine_mod():
     return engine
 
 
-def _setup(N=2048, precision="fp64", seed=7, np_seed=3, workload="single_dust", craft=True):
+MODES = {"rows": 1, "sym": 2}
+
+
+def _setup(N=2048, precision="fp64", seed=7, np_seed=3, workload="single_dust", craft=True, mode="rows"):
     from opendust_amd import workloads
 
     od = getattr(workloads, workload)(N=N, n=100, precision=precision, seed=seed)
     alpha = workloads.synthetic_alpha(od)
     cfg, x0, v0 = workloads.prepare(od, alpha=alpha, np_seed=np_seed, precision=precision)
+    cfg.pair_mode = MODES[mode]
     if craft:
         # ions 0..5 inside the dust(s), 6..11 outside the side wall, 12..17
         # beyond the end caps: they are absorbed / flagged this step
@@ -62,12 +66,13 @@ def test_library_loads_on_gpu():
     assert eng.check_abi()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_pair_forces_vs_oracle(oracle_lib, precision):
+@pytest.mark.parametrize("precision,mode", [("fp64", "rows"), ("fp32c", "rows"), ("fp32c", "sym")])
+def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
     eng = _engine_mod()
-    od, cfg, x0, v0 = _setup(N=3000, precision=precision, craft=False)
-    say("pair forces", precision, "N", cfg.n_ions)
+    od, cfg, x0, v0 = _setup(N=3000, precision=precision, craft=False, mode=mode)
+    say("pair forces", precision, mode, "N", cfg.n_ions)
     e = eng.Engine(cfg)
+    assert e.pair_info()[0] == MODES[mode]
     f_gpu = e.pair_forces(x0)
     e.close()
     f_orc = oracle_lib.pair_forces(cfg, x0)
@@ -83,12 +88,12 @@ def test_pair_forces_vs_oracle(oracle_lib, precision):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_one_step_vs_oracle(oracle_lib, precision):
+@pytest.mark.parametrize("precision,mode", [("fp64", "rows"), ("fp32c", "rows"), ("fp32c", "sym")])
+def test_one_step_vs_oracle(oracle_lib, precision, mode):
     eng = _engine_mod()
-    od, cfg, x0, v0 = _setup(N=2048, precision=precision)
+    od, cfg, x0, v0 = _setup(N=2048, precision=precision, mode=mode)
     q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
-    say("one step", precision, "N", cfg.n_ions)
+    say("one step", precision, mode, "N", cfg.n_ions)
     e = eng.Engine(cfg)
     e.set_ions(x0, v0)
     e.set_step(5)
@@ -134,10 +139,11 @@ def test_multi_step_fp64_records(oracle_lib):
     assert np.max(np.abs(x_g - x_o)) <= 1e-10 * cfg.R
 
 
-def test_determinism_and_graph(oracle_lib):
+@pytest.mark.parametrize("mode", ["rows", "sym"])
+def test_determinism_and_graph(oracle_lib, mode):
     eng = _engine_mod()
-    od, cfg, x0, v0 = _setup(N=4096, precision="fp32c", craft=True)
-    say("determinism fp32c N", cfg.n_ions)
+    od, cfg, x0, v0 = _setup(N=4096, precision="fp32c", craft=True, mode=mode)
+    say("determinism fp32c", mode, "N", cfg.n_ions)
     outs = []
     for flags in (0, 0, eng.RUN_GRAPH):
         e = eng.Engine(cfg)
