@@ -78,6 +78,36 @@ def cpu_baseline(cfg, x, target_s=12.0):
     }
 
 
+def cutoff_leg(steps):
+    """2^22 superions, cylinder R = 10, H = 20 r_D_e, ion-ion force truncated
+    at 2 r_D_e through the per-step cell list; in-range pair interactions/s."""
+    from opendust_amd import engine, workloads
+
+    od = workloads.cutoff_large(N=2 ** 22, n=steps + 1, seed=99, trapSeed=7)
+    cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=11, precision="fp32c")
+    e = engine.Engine(cfg)
+    e.set_ions(x0, v0)
+    e.run(1, engine.RUN_PROFILE_PAIR)
+    e.pair_timing(reset=True)
+    t0 = time.perf_counter()
+    e.run(steps, engine.RUN_PROFILE_PAIR)
+    wall = time.perf_counter() - t0
+    ms, nl = e.pair_timing()
+    in_range, cand = e.cutoff_pair_count()
+    e.close()
+    return {
+        "workload": "single dust, 2^22 superions, R=10 r_D_e, H=20 r_D_e, hard cutoff 2 r_D_e, cell list rebuilt "
+                    "every step",
+        "steps": steps,
+        "ms_per_step": wall / steps * 1e3,
+        "in_range_pairs_per_step": in_range,
+        "candidate_pairs_per_step": cand,
+        "pair_interactions_per_s": in_range * steps / wall,
+        "cell_kernel_ms": ms / max(nl, 1),
+        "cell_kernel_tflops_alg": in_range * FLOP_PER_PAIR / (ms / max(nl, 1) * 1e-3) / 1e12,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +119,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pair-only", type=int, default=0)
     ap.add_argument("--pair-mode", choices=["sym", "rows", "auto"], default="sym")
+    ap.add_argument("--cutoff-steps", type=int, default=0,
+                    help="also time the cutoff workload (2^22 ions, R=10, H=20 r_D_e, r_c=2 r_D_e)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -168,6 +200,12 @@ def main():
         }
         log("fp64:", fp64)
 
+    cut = None
+    if args.cutoff_steps > 0:
+        log("cutoff workload")
+        cut = cutoff_leg(args.cutoff_steps)
+        log("cutoff:", cut)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -210,6 +248,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
+        "cutoff_2e22": cut,
         "absorbed_last_step": int(rec["absorbed"][-1, 0]),
     }
     print(json.dumps(out))

@@ -49,6 +49,7 @@ extern "C" {
 #define OD_PAIR_AUTO 0 /* symmetric when the problem is large enough, else rows */
 #define OD_PAIR_ROWS 1 /* every ordered pair, one target row per thread */
 #define OD_PAIR_SYM 2  /* every unordered pair once (Newton's third law) */
+#define OD_PAIR_CELL 3 /* reported by od_pair_info when a cutoff is set: cell list + stencil */
 
 /* charge methods (opendust.py:1120, :1153; any other string keeps q) */
 #define OD_CHARGE_GIVEN 0
@@ -73,6 +74,7 @@ typedef struct od_table {
 /* Everything the step needs; filled by the Python host layer. */
 typedef struct od_config {
   int32_t abi_version;   /* = OD_ABI_VERSION */
+  int32_t struct_size;   /* = sizeof(od_config) as the caller laid it out */
   int32_t n_ions;        /* N superions */
   int32_t n_dust;        /* 0 .. OD_MAX_DUST */
   int32_t distribution;  /* OD_DIST_* */
@@ -82,6 +84,7 @@ typedef struct od_config {
   int32_t j_split;       /* source chunks of the row pair kernel; 0 = automatic */
   int32_t pair_mode;     /* OD_PAIR_*: which all-pairs kernel */
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
+  int32_t reserved_;
   uint64_t seed;         /* Philox4x32-10 key */
 
   /* geometry, integration (OpenMM units) */
@@ -96,6 +99,17 @@ typedef struct od_config {
   double trap_C;         /* prefactor of the trap polynomial, kJ/mol (opendust.py:552-557) */
   double trap_alpha[15]; /* fitted coefficients alphaTrap[0..14] (opendust.py:547) */
   double field_force;    /* E*q_s*conv, kJ/mol/nm (fieldDriven, opendust.py:580-585) */
+
+  /* Optional truncation of the ion-ion law (opendust.py:481-485 uses OpenMM's
+   * CutoffNonPeriodic with cutoff 3.2 r_D_e and a switch from 3 r_D_e).
+   * cutoff <= 0: all pairs.  switch_dist in (0, cutoff): the energy is
+   * multiplied by S(x) = 1 - 10x^3 + 15x^4 - 6x^5, x = (r - switch)/(cutoff - switch),
+   * and the force is -d(U S)/dr; otherwise hard truncation at r < cutoff.
+   * Sources are found through a cell list with cells of cutoff/2 (nm):
+   *   nx = ny = ceil(2R/h), nz = ceil(H/h), ix = clamp(floor((x + R)/h), 0, nx-1),
+   *   iy likewise, iz = clamp(floor((z + H/2)/h), 0, nz-1), cell = ix + nx (iy + ny iz). */
+  double cutoff;         /* nm */
+  double switch_dist;    /* nm */
 
   /* Philox decision thresholds on a u32 draw, precomputed from p_t, p_s, p_b, dt*w_c */
   uint64_t thr_top;      /* face = top    if u32 <  thr_top */
@@ -177,6 +191,17 @@ int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t res
  * times) covers: N(N-1) for rows, N(N-1) minus the intra-block pairs for the
  * symmetric kernel.  No counterpart in the reference (measurement support). */
 int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
+
+/* Cutoff mode only: the cell list of the last force evaluation.  cell_of_ion[n]
+ * (binned ions: their cell; ions left out: the number of cells), sorted_ions
+ * (binned ions in cell order, ties by ion index), their count, and the grid
+ * dims[3].  Any pointer may be NULL.  No counterpart in the reference (OpenMM
+ * keeps its neighbour list internal). */
+int od_get_cells(od_ctx* ctx, int32_t* cell_of_ion, int32_t* sorted_ions, int64_t* n_binned, int32_t* dims);
+
+/* Cutoff mode only: ordered ion pairs within the cutoff and candidate pairs
+ * visited by the stencil, for the current cell list (measurement support). */
+int od_cutoff_pair_count(od_ctx* ctx, int64_t* in_range, int64_t* candidates);
 
 /* Diagnostics of the last step (for parity tests): per-ion flag word
  * (bit p < 16: inside dust p; bit 16: outside in r; bit 17: outside in z),

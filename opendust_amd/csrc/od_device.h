@@ -144,6 +144,61 @@ __device__ __forceinline__ void pair64(double xi, double yi, double zi, double x
   az = fma(h, dz, az);
 }
 
+// Truncated / switched forms.  Inside the cutoff the energy is U(r) S(x) with
+// S = 1 - 10x^3 + 15x^4 - 6x^5, x = (r - r_s)/(r_c - r_s) clamped to [0, 1]; the
+// force is -(U' S + U S').  In the FP32 units above, with e = 2^-r and w as in
+// pair32, U S'/r contributes -e w dS/dr (dS/dr in the same units), so
+// h -> h S - e w S'.  Beyond the cutoff h = 0 (no branch: a select).
+template <bool SW>
+__device__ __forceinline__ void pair32_cut(float xi, float yi, float zi, float xj, float yj, float zj, float a,
+                                           float linv, float rc, float rs, float invw, float& ax, float& ay,
+                                           float& az) {
+  const float dx = xj - xi;
+  const float dy = yj - yi;
+  const float dz = zj - zi;
+  const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32)));
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  const float e = __builtin_amdgcn_exp2f(-r);
+  const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(a, r, r2));
+  const float t = __builtin_fmaf(r, w, linv);
+  float h = e * w * t;
+  if constexpr (SW) {
+    const float x = __builtin_fminf(__builtin_fmaxf((r - rs) * invw, 0.0f), 1.0f);
+    const float x2 = x * x;
+    const float S = __builtin_fmaf(x2 * x, __builtin_fmaf(x, __builtin_fmaf(x, -6.0f, 15.0f), -10.0f), 1.0f);
+    const float omx = 1.0f - x;
+    const float dS = -30.0f * x2 * omx * omx * invw;
+    h = __builtin_fmaf(h, S, -(e * w * dS));
+  }
+  h = (r < rc) ? h : 0.0f;
+  ax = __builtin_fmaf(h, dx, ax);
+  ay = __builtin_fmaf(h, dy, ay);
+  az = __builtin_fmaf(h, dz, az);
+}
+
+__device__ __forceinline__ void pair64_cut(double xi, double yi, double zi, double xj, double yj, double zj,
+                                           double rho0, bool sw, double rc, double rs, double invw, double& ax,
+                                           double& ay, double& az) {
+  const double dx = xj - xi;
+  const double dy = yj - yi;
+  const double dz = zj - zi;
+  const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, EPS64)));
+  const double r = sqrt(r2);
+  if (!(r < rc)) return;
+  const double e = exp(-r);
+  const double w = 1.0 / (r * (r + rho0));
+  double h = e * w * fma(r, w, 1.0);
+  if (sw && r > rs) {
+    const double x = (r - rs) * invw;
+    const double S = 1.0 - 10.0 * x * x * x + 15.0 * x * x * x * x - 6.0 * x * x * x * x * x;
+    const double dS = (-30.0 * x * x + 60.0 * x * x * x - 30.0 * x * x * x * x) * invw;
+    h = h * S - e * w * dS;
+  }
+  ax = fma(h, dx, ax);
+  ay = fma(h, dy, ay);
+  az = fma(h, dz, az);
+}
+
 // Kahan step: add a block partial p into the compensated sum (s, c).
 __device__ __forceinline__ void kahan_add(float& s, float& c, float p) {
   const float y = p - c;

@@ -75,6 +75,13 @@ struct DevParams {
   double pair_scale;  // kernel accumulator -> kJ/mol/nm
   float a32, linv32;
   double rho0;
+  // truncation (cutoff mode): FP32 values in L/lambda units, FP64 in lambda units
+  int use_cut, use_switch;
+  float rc32, rs32, invw32;
+  double rc64, rs64, invw64;
+  // cell grid (nm)
+  double cell_h;
+  int nx, ny, nz, n_cells;
   uint64_t thr_top, thr_side, thr_coll;
   int dust_method[OD_MAX_DUST];
   double si_k, si_e, si_q_s, si_lam, si_dt, si_kB, si_Te, si_Tn, si_E;
@@ -125,12 +132,21 @@ __device__ __forceinline__ void pair_acc(const DevParams& P, const typename Pos<
 template <>
 __device__ __forceinline__ void pair_acc<float>(const DevParams& P, const float4& i, const float4& j, Acc<float>& ax,
                                                 Acc<float>& ay, Acc<float>& az) {
-  pair32(i.x, i.y, i.z, j.x, j.y, j.z, P.a32, P.linv32, ax.a, ay.a, az.a);
+  if (!P.use_cut)
+    pair32(i.x, i.y, i.z, j.x, j.y, j.z, P.a32, P.linv32, ax.a, ay.a, az.a);
+  else if (P.use_switch)
+    pair32_cut<true>(i.x, i.y, i.z, j.x, j.y, j.z, P.a32, P.linv32, P.rc32, P.rs32, P.invw32, ax.a, ay.a, az.a);
+  else
+    pair32_cut<false>(i.x, i.y, i.z, j.x, j.y, j.z, P.a32, P.linv32, P.rc32, P.rs32, P.invw32, ax.a, ay.a, az.a);
 }
 template <>
 __device__ __forceinline__ void pair_acc<double>(const DevParams& P, const double4& i, const double4& j,
                                                  Acc<double>& ax, Acc<double>& ay, Acc<double>& az) {
-  pair64(i.x, i.y, i.z, j.x, j.y, j.z, P.rho0, ax.a, ay.a, az.a);
+  if (!P.use_cut)
+    pair64(i.x, i.y, i.z, j.x, j.y, j.z, P.rho0, ax.a, ay.a, az.a);
+  else
+    pair64_cut(i.x, i.y, i.z, j.x, j.y, j.z, P.rho0, P.use_switch != 0, P.rc64, P.rs64, P.invw64, ax.a, ay.a,
+               az.a);
 }
 
 // deterministic block sum of a double over a 256-thread block (value valid on all threads)
@@ -962,6 +978,268 @@ __global__ __launch_bounds__(256) void k_trap_finish(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Cutoff mode: cell list built every step by integer binning and a stable,
+// deterministic LSD radix sort (8-bit digits, two passes for 16-bit cell ids;
+// within a digit, items keep their input order), then a pair kernel over the
+// 5x5x5 cell stencil (cells of cutoff/2).
+// ---------------------------------------------------------------------------
+constexpr int RS_ITEMS = 1024;  // items per radix-sort block (256 threads x 4 rounds)
+
+__device__ __forceinline__ int cell_of(const DevParams& P, double x, double y, double z) {
+  int ix = (int)floor((x + P.R) / P.cell_h);
+  int iy = (int)floor((y + P.R) / P.cell_h);
+  int iz = (int)floor((z + P.half_H) / P.cell_h);
+  ix = min(max(ix, 0), P.nx - 1);
+  iy = min(max(iy, 0), P.ny - 1);
+  iz = min(max(iz, 0), P.nz - 1);
+  return ix + P.nx * (iy + P.ny * iz);
+}
+
+// key = cell of the ion (FP64 position), or n_cells for ions left out (flagged,
+// or padding); value = ion index
+__global__ __launch_bounds__(256) void k_bin(const DevParams* __restrict__ P, const double* __restrict__ X,
+                                             const uint32_t* __restrict__ flags, uint32_t* __restrict__ key,
+                                             uint32_t* __restrict__ val) {
+  const int np = P->n_pad;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= np) return;
+  uint32_t k = (uint32_t)P->n_cells;
+  if (i < P->n && (flags == nullptr || flags[i] == 0u)) k = (uint32_t)cell_of(*P, X[i], X[np + i], X[2 * np + i]);
+  key[i] = k;
+  val[i] = (uint32_t)i;
+}
+
+// per-block digit histogram, digit-major: hist[d * nblk + b]
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ key, int n, int shift, int nblk,
+                                                 int* __restrict__ hist) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RS_ITEMS;
+  for (int r = 0; r < RS_ITEMS / 256; ++r) {
+    const int i = base + r * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// stable scatter: rank = (digit base of this block) + (same-digit items earlier in the block)
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ key_in,
+                                                    const uint32_t* __restrict__ val_in, int n, int shift, int nblk,
+                                                    const int* __restrict__ off, uint32_t* __restrict__ key_out,
+                                                    uint32_t* __restrict__ val_out) {
+  __shared__ int run[256];
+  __shared__ int wc[4][256];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  run[t] = off[t * nblk + blockIdx.x];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int base = blockIdx.x * RS_ITEMS;
+  for (int r = 0; r < RS_ITEMS / 256; ++r) {
+    for (int w = 0; w < 4; ++w) wc[w][t] = 0;
+    __syncthreads();
+    const int i = base + r * 256 + t;
+    const bool valid = i < n;
+    const uint32_t k = valid ? key_in[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const unsigned long long m = __ballot(valid && ((d >> b) & 1u));
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pos = run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wc[w][d];
+      key_out[pos] = k;
+      val_out[pos] = val_in[i];
+    }
+    __syncthreads();
+    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, int n, int v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ key, int n, int n_cells,
+                                                    int* __restrict__ count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && (int)key[i] < n_cells) atomicAdd(&count[key[i]], 1);
+}
+
+__global__ __launch_bounds__(256) void k_cell_chunks(const int* __restrict__ count, int n_cells,
+                                                     int* __restrict__ chunks) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < n_cells) chunks[c] = (count[c] + 63) / 64;
+}
+
+// work items (cell, 64-target chunk), packed as cell | chunk << 16
+__global__ __launch_bounds__(256) void k_cell_items(const int* __restrict__ chunks, const int* __restrict__ item_off,
+                                                    int n_cells, int* __restrict__ items) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n_cells) return;
+  const int o = item_off[c];
+  for (int k = 0; k < chunks[c]; ++k) items[o + k] = c | (k << 16);
+}
+
+// sorted source positions (kept ions) and their ion ids
+template <class T>
+__global__ __launch_bounds__(256) void k_cell_gather(const uint32_t* __restrict__ skey,
+                                                     const uint32_t* __restrict__ sval, int n_pad, int n_cells,
+                                                     const typename Pos<T>::V* __restrict__ pos,
+                                                     typename Pos<T>::V* __restrict__ spos) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n_pad) return;
+  spos[p] = ((int)skey[p] < n_cells) ? pos[sval[p]] : Pos<T>::park();
+}
+
+// Forces on one wave's 64 targets (one cell, one chunk) from the 5x5x5 stencil;
+// sources are wave-uniform (scalar loads), pairs beyond the cutoff are masked.
+template <class T>
+__global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__ P,
+                                                   const typename Pos<T>::V* __restrict__ spos,
+                                                   const uint32_t* __restrict__ sval,
+                                                   const int* __restrict__ cell_start, const int* __restrict__ items,
+                                                   const int* __restrict__ n_items, double* __restrict__ F) {
+  const int lane = threadIdx.x & 63;
+  const int np = P->n_pad;
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const int nit = n_items[0];
+  const int nw = gridDim.x * 4;
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nit; it += nw) {
+    const int packed = items[it];
+    const int c = packed & 0xFFFF, k = packed >> 16;
+    const int t0 = cell_start[c] + k * 64;
+    const int tp = t0 + lane;
+    const bool valid = tp < cell_start[c + 1];
+    const typename Pos<T>::V own = valid ? spos[tp] : Pos<T>::park();
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    Acc<T> ax, ay, az;
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z) {
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
+        const int row = nx * (y + ny * z);
+        const int j0 = cell_start[row + max(cx - 2, 0)];
+        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
+        for (int jt = j0; jt < j1; jt += TJ) {
+          const int je = min(jt + TJ, j1);
+          for (int j = jt; j < je; ++j) pair_acc<T>(*P, own, spos[j], ax, ay, az);
+          ax.flush();
+          ay.flush();
+          az.flush();
+        }
+      }
+    }
+    if (valid) {
+      const int i = (int)sval[tp];
+      F[i] = ax.total() * P->pair_scale;
+      F[np + i] = ay.total() * P->pair_scale;
+      F[2 * np + i] = az.total() * P->pair_scale;
+    }
+  }
+}
+
+// Rows of listed ions in cutoff mode: kept sources from the cell stencil of the
+// ion's cell (FP64 position X) plus the other listed ions `others`.
+template <class T>
+__global__ __launch_bounds__(256) void k_rows_cell(const DevParams* __restrict__ P, const double* __restrict__ X,
+                                                   const typename Pos<T>::V* __restrict__ spos,
+                                                   const int* __restrict__ cell_start, const int* __restrict__ n_flag,
+                                                   const typename Pos<T>::V* __restrict__ own_list,
+                                                   const typename Pos<T>::V* __restrict__ others,
+                                                   const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
+                                                   uint32_t need_mask, double* __restrict__ rowF) {
+  __shared__ double lds[4];
+  const int nf = n_flag[0];
+  const int np = P->n_pad;
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  for (int s = blockIdx.x; s < nf; s += gridDim.x) {
+    const int ion = list_ion[s];
+    if (need_mask != 0u && (flags[ion] & need_mask) == 0u) continue;
+    const typename Pos<T>::V own = own_list[s];
+    const int c = cell_of(*P, X[ion], X[np + ion], X[2 * np + ion]);
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    Acc<T> ax, ay, az;
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z) {
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
+        const int row = nx * (y + ny * z);
+        const int j0 = cell_start[row + max(cx - 2, 0)];
+        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
+        for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) pair_acc<T>(*P, own, spos[j], ax, ay, az);
+        ax.flush();
+        ay.flush();
+        az.flush();
+      }
+    }
+    for (int j = threadIdx.x; j < nf; j += 256) pair_acc<T>(*P, own, others[j], ax, ay, az);
+    ax.flush();
+    ay.flush();
+    az.flush();
+    const double fx = block_sum256(ax.total(), lds);
+    const double fy = block_sum256(ay.total(), lds);
+    const double fz = block_sum256(az.total(), lds);
+    if (threadIdx.x == 0) {
+      rowF[3 * s + 0] = fx * P->pair_scale;
+      rowF[3 * s + 1] = fy * P->pair_scale;
+      rowF[3 * s + 2] = fz * P->pair_scale;
+    }
+  }
+}
+
+// in-range and candidate pair counts over the current cell list (the same
+// distance test as k_pair_cell); integer atomics, so the totals are exact
+template <class T>
+__global__ __launch_bounds__(256) void k_cell_pair_count(const DevParams* __restrict__ P,
+                                                         const typename Pos<T>::V* __restrict__ spos,
+                                                         const int* __restrict__ cell_start,
+                                                         const int* __restrict__ items, const int* __restrict__ n_items,
+                                                         unsigned long long* __restrict__ counts) {
+  const int lane = threadIdx.x & 63;
+  const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const int nit = n_items[0];
+  const int nw = gridDim.x * 4;
+  unsigned long long in_range = 0, cand = 0;
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nit; it += nw) {
+    const int packed = items[it];
+    const int c = packed & 0xFFFF, k = packed >> 16;
+    const int tp = cell_start[c] + k * 64 + lane;
+    const bool valid = tp < cell_start[c + 1];
+    const typename Pos<T>::V own = valid ? spos[tp] : Pos<T>::park();
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z)
+      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
+        const int row = nx * (y + ny * z);
+        const int j0 = cell_start[row + max(cx - 2, 0)];
+        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
+        for (int j = j0; j < j1; ++j) {
+          if (!valid || j == tp) continue;
+          const typename Pos<T>::V q = spos[j];
+          bool in;
+          if constexpr (sizeof(T) == 4) {
+            const float dx = q.x - own.x, dy = q.y - own.y, dz = q.z - own.z;
+            const float r = __builtin_amdgcn_sqrtf(
+                __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32))));
+            in = r < P->rc32;
+          } else {
+            const double dx = q.x - own.x, dy = q.y - own.y, dz = q.z - own.z;
+            in = sqrt(fma(dx, dx, fma(dy, dy, fma(dz, dz, EPS64)))) < P->rc64;
+          }
+          in_range += in ? 1 : 0;
+          cand += 1;
+        }
+      }
+  }
+  atomicAdd(&counts[0], in_range);
+  atomicAdd(&counts[1], cand);
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 struct od_ctx {
@@ -994,6 +1272,12 @@ struct od_ctx {
   int* wg_base = nullptr;
   double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
   float* sym_sp = nullptr;
+  // cutoff mode: cell list
+  int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
+  uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
+  int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_count = nullptr, *cell_start = nullptr,
+      *cell_chunks = nullptr, *item_off = nullptr, *items = nullptr, *n_items = nullptr;
+  void* spos = nullptr;
 };
 
 static int fill_params(const od_config* c, int n_pad, DevParams* P) {
@@ -1023,6 +1307,22 @@ static int fill_params(const od_config* c, int n_pad, DevParams* P) {
   } else {
     P->pos_scale = 1.0 / c->r_D_e;
     P->pair_scale = -C;
+  }
+  if (c->cutoff > 0.0) {
+    P->use_cut = 1;
+    P->use_switch = (c->switch_dist > 0.0 && c->switch_dist < c->cutoff) ? 1 : 0;
+    const double rc = c->cutoff / c->r_D_e, rs = P->use_switch ? c->switch_dist / c->r_D_e : rc;
+    P->rc64 = rc;
+    P->rs64 = rs;
+    P->invw64 = P->use_switch ? 1.0 / (rc - rs) : 0.0;
+    P->rc32 = (float)(rc * L);
+    P->rs32 = (float)(rs * L);
+    P->invw32 = P->use_switch ? (float)(1.0 / ((rc - rs) * L)) : 0.0f;
+    P->cell_h = c->cutoff / 2.0;
+    P->nx = (int)ceil(2.0 * c->R / P->cell_h);
+    P->ny = P->nx;
+    P->nz = (int)ceil(c->H / P->cell_h);
+    P->n_cells = P->nx * P->ny * P->nz;
   }
   P->a32 = (float)(P->rho0 * L);
   P->linv32 = (float)(1.0 / L);
@@ -1079,7 +1379,9 @@ static void free_ctx(od_ctx* c) {
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
-                  c->rowD, c->sym_sp};
+                  c->rowD, c->sym_sp, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
+                  c->rs_off, c->rs_total, c->cell_count, c->cell_start, c->cell_chunks, c->item_off, c->items,
+                  c->n_items, c->spos};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -1119,9 +1421,33 @@ static int64_t sym_wg_count(int n_blocks, int ch) {
   return n;
 }
 
+static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
+  c->cut = 0;
+  if (!(cfg->cutoff > 0.0)) return OD_OK;
+  if ((double)hp.nx * hp.ny * hp.nz > 65535.0)
+    return set_err(OD_ERR_ARG, "cutoff %g nm gives %d x %d x %d cells; at most 65535 are supported", cfg->cutoff,
+                   hp.nx, hp.ny, hp.nz);
+  c->cut = 1;
+  c->n_cells = hp.n_cells;
+  c->rs_nblk = (c->n_pad + RS_ITEMS - 1) / RS_ITEMS;
+  c->max_items = c->n_pad / 64 + c->n_cells + 1;
+  const size_t np = c->n_pad;
+  int rc = OD_OK;
+  if ((rc = dalloc(&c->ckey[0], np * 4)) || (rc = dalloc(&c->ckey[1], np * 4)) || (rc = dalloc(&c->cval[0], np * 4)) ||
+      (rc = dalloc(&c->cval[1], np * 4)) || (rc = dalloc(&c->rs_hist, (size_t)256 * c->rs_nblk * 4)) ||
+      (rc = dalloc(&c->rs_off, (size_t)256 * c->rs_nblk * 4)) || (rc = dalloc(&c->rs_total, 4)) ||
+      (rc = dalloc(&c->cell_count, (size_t)(c->n_cells + 1) * 4)) ||
+      (rc = dalloc(&c->cell_start, (size_t)(c->n_cells + 1) * 4)) ||
+      (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cells + 1) * 4)) ||
+      (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * 4)) ||
+      (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)))
+    return rc;
+  return OD_OK;
+}
+
 static int setup_sym(od_ctx* c, const od_config* cfg) {
   c->sym = 0;
-  if (c->prec != OD_PREC_FP32C || cfg->pair_mode == OD_PAIR_ROWS) return OD_OK;
+  if (c->prec != OD_PREC_FP32C || cfg->pair_mode == OD_PAIR_ROWS || cfg->cutoff > 0.0) return OD_OK;
   const int T = c->n_pad / SYM_TB;
   const bool forced = cfg->pair_mode == OD_PAIR_SYM;
   if (T < 2 || (!forced && c->n_pad < (1 << 17))) return OD_OK;
@@ -1160,6 +1486,9 @@ static int check_config(const od_config* c) {
   if (!c) return set_err(OD_ERR_ARG, "null config");
   if (c->abi_version != OD_ABI_VERSION)
     return set_err(OD_ERR_ARG, "ABI version %d, library expects %d", c->abi_version, OD_ABI_VERSION);
+  if (c->struct_size != (int32_t)sizeof(od_config))
+    return set_err(OD_ERR_ARG, "od_config layout mismatch: caller %d bytes, library %d bytes", c->struct_size,
+                   (int)sizeof(od_config));
   if (c->n_ions < 1) return set_err(OD_ERR_ARG, "n_ions must be >= 1 (got %d)", c->n_ions);
   if (c->n_ions > (1 << 28)) return set_err(OD_ERR_ARG, "n_ions %d above the supported 2^28", c->n_ions);
   if (c->n_dust < 0 || c->n_dust > OD_MAX_DUST)
@@ -1223,6 +1552,7 @@ extern "C" int od_create(const od_config* cfg, od_ctx** out) {
     return fail(rc);
   DevParams hp;
   fill_params(cfg, c->n_pad, &hp);
+  if ((rc = setup_cells(c, cfg, hp))) return fail(rc);
   double q0[OD_MAX_DUST] = {0};
   for (int p = 0; p < cfg->n_dust; ++p) q0[p] = cfg->dust_q0[p];
   uint32_t st0[4] = {0, 0, 0, 0};
@@ -1282,6 +1612,56 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
   return OD_OK;
 }
 
+// cell list of the ions at FP64 positions X (flagged ions left out when flags != null)
+template <class T>
+static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, const void* pos) {
+  using PV = typename Pos<T>::V;
+  const int np = c->n_pad, nb = c->nb, nc = c->n_cells;
+  hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, c->stream, c->dP, X, flags, c->ckey[0], c->cval[0]);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int in = pass, out = 1 - pass, shift = 8 * pass;
+    hipLaunchKernelGGL(k_rs_hist, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], np, shift, c->rs_nblk,
+                       c->rs_hist);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->rs_hist, 256 * c->rs_nblk, c->rs_off,
+                       c->rs_total);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], c->cval[in], np, shift,
+                       c->rs_nblk, c->rs_off, c->ckey[out], c->cval[out]);
+  }
+  const int gc = (nc + 256) / 256;
+  hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, c->stream, c->cell_count, nc + 1, 0);
+  hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, c->stream, c->ckey[0], np, nc, c->cell_count);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->cell_count, nc, c->cell_start,
+                     c->cell_start + nc);
+  hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_count, nc, c->cell_chunks);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->cell_chunks, nc, c->item_off, c->n_items);
+  hipLaunchKernelGGL(k_cell_items, dim3(gc), dim3(256), 0, c->stream, c->cell_chunks, c->item_off, nc, c->items);
+  hipLaunchKernelGGL(k_cell_gather<T>, dim3(nb), dim3(256), 0, c->stream, c->ckey[0], c->cval[0], np, nc,
+                     (const PV*)pos, (PV*)c->spos);
+  return OD_OK;
+}
+
+// Ion-ion force on every ion from every other (kept) ion: the all-pairs kernels,
+// or, in cutoff mode, the cell list + stencil kernel.  X: FP64 positions of the
+// same state as pos; flags: ions to leave out (cutoff mode only; may be null).
+template <class T>
+static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const uint32_t* flags, double* F,
+                           bool timed) {
+  using PV = typename Pos<T>::V;
+  if (!c->cut) return launch_pair(c, pos, pos, F, timed);
+  CHKRET(build_cells_t<T>(c, X, flags, pos));
+  const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
+  if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_pair_cell<T>, dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos, c->cval[0],
+                     c->cell_start, c->items, c->n_items, F);
+  if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+  return OD_OK;
+}
+
+static int launch_forces(od_ctx* c, const double* X, const void* pos, const uint32_t* flags, double* F, bool timed) {
+  return (c->prec == OD_PREC_FP32C) ? launch_forces_t<float>(c, X, pos, flags, F, timed)
+                                    : launch_forces_t<double>(c, X, pos, flags, F, timed);
+}
+
 template <class T>
 static int launch_step_t(od_ctx* c, bool timed) {
   using PV = typename Pos<T>::V;
@@ -1291,17 +1671,29 @@ static int launch_step_t(od_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->block_cnt, nb, c->block_off, c->n_flag);
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, c->stream, c->flags, c->block_cnt, c->block_off, c->n,
                      (const PV*)c->pi, c->slot_of, c->list_ion, (PV*)c->list_old);
-  // flagged ions are parked in pj, as sources and as targets
-  CHKRET(launch_pair(c, c->pj, c->pj, c->F, timed));
+  // flagged ions are parked in pj, as sources and as targets (cutoff mode:
+  // left out of the cell list)
+  CHKRET(launch_forces_t<T>(c, c->X, c->pj, c->flags, c->F, timed));
   const uint32_t dust_bits = (c->cfg.n_dust > 0) ? ((1u << c->cfg.n_dust) - 1u) : 0u;
-  if (dust_bits)
-    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pi, c->n_flag,
-                       (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
+  if (dust_bits) {
+    if (c->cut)
+      hipLaunchKernelGGL(k_rows_cell<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, c->X,
+                         (const PV*)c->spos, c->cell_start, c->n_flag, (const PV*)c->list_old,
+                         (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
+    else
+      hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pi, c->n_flag,
+                         (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
+  }
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
                      (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
                      (PV*)c->list_new, c->face, c->rowD, c->partials);
-  hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pj, c->n_flag,
-                     (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
+  if (c->cut)
+    hipLaunchKernelGGL(k_rows_cell<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, c->X, (const PV*)c->spos,
+                       c->cell_start, c->n_flag, (const PV*)c->list_new, (const PV*)c->list_new, c->flags,
+                       c->list_ion, 0u, c->rowF);
+  else
+    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pj, c->n_flag,
+                       (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->F, (const PV*)c->pj, c->flags,
                      c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
   hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, nb, c->q, c->st, c->rec);
@@ -1343,7 +1735,7 @@ extern "C" int od_set_ions(od_ctx* c, const double* x, const double* v, int64_t 
   HIPCHK(hipMemcpyAsync(c->X, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->V, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, c->stream));
   CHKRET(load_positions(c, c->X, c->pi, c->pj));
-  CHKRET(launch_pair(c, c->pj, c->pj, c->F, false));
+  CHKRET(launch_forces(c, c->X, c->pj, nullptr, c->F, false));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return OD_OK;
@@ -1469,10 +1861,70 @@ extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t fl
   return OD_OK;
 }
 
+extern "C" int od_get_cells(od_ctx* c, int32_t* cell_of_ion, int32_t* sorted_ions, int64_t* n_binned,
+                            int32_t* dims) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (!c->cut) return set_err(OD_ERR_STATE, "od_get_cells: context has no cutoff (no cell list)");
+  HIPCHK(hipSetDevice(c->dev));
+  std::vector<uint32_t> key(c->n_pad), val(c->n_pad);
+  HIPCHK(hipMemcpyAsync(key.data(), c->ckey[0], c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(val.data(), c->cval[0], c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int64_t nb = 0;
+  for (int p = 0; p < c->n_pad; ++p) {
+    const uint32_t i = val[p];
+    if ((int)i < c->n && cell_of_ion) cell_of_ion[i] = (int32_t)key[p];
+    if ((int)key[p] < c->n_cells) {
+      if (sorted_ions) sorted_ions[nb] = (int32_t)i;
+      ++nb;
+    }
+  }
+  if (n_binned) *n_binned = nb;
+  if (dims) {
+    DevParams hp;
+    fill_params(&c->cfg, c->n_pad, &hp);
+    dims[0] = hp.nx;
+    dims[1] = hp.ny;
+    dims[2] = hp.nz;
+  }
+  return OD_OK;
+}
+
+extern "C" int od_cutoff_pair_count(od_ctx* c, int64_t* in_range, int64_t* candidates) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (!c->cut) return set_err(OD_ERR_STATE, "od_cutoff_pair_count: context has no cutoff");
+  HIPCHK(hipSetDevice(c->dev));
+  unsigned long long* d = nullptr;
+  CHKRET(dalloc(&d, 16));
+  unsigned long long h[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(d, h, 16, hipMemcpyHostToDevice, c->stream);
+  const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
+  if (e == hipSuccess) {
+    if (c->prec == OD_PREC_FP32C)
+      hipLaunchKernelGGL(k_cell_pair_count<float>, dim3(grid), dim3(256), 0, c->stream, c->dP,
+                         (const float4*)c->spos, c->cell_start, c->items, c->n_items, d);
+    else
+      hipLaunchKernelGGL(k_cell_pair_count<double>, dim3(grid), dim3(256), 0, c->stream, c->dP,
+                         (const double4*)c->spos, c->cell_start, c->items, c->n_items, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_cutoff_pair_count: %s", hipGetErrorString(e));
+  if (in_range) *in_range = (int64_t)h[0];
+  if (candidates) *candidates = (int64_t)h[1];
+  return OD_OK;
+}
+
 extern "C" int od_pair_info(od_ctx* c, int32_t* mode, int64_t* pairs_per_launch) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   const int64_t n = c->n;
-  if (mode) *mode = c->sym ? OD_PAIR_SYM : OD_PAIR_ROWS;
+  if (mode) *mode = c->cut ? OD_PAIR_CELL : (c->sym ? OD_PAIR_SYM : OD_PAIR_ROWS);
+  if (c->cut) {
+    if (pairs_per_launch) *pairs_per_launch = -1;  // data dependent: od_cutoff_pair_count
+    return OD_OK;
+  }
   if (pairs_per_launch) {
     if (c->sym) {
       // ordered pairs between different blocks (the main kernel's share)
@@ -1533,7 +1985,7 @@ extern "C" int od_pair_forces(od_ctx* c, const double* x, double* f) {
   hipError_t e = hipMemcpyAsync(dX, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     rc = load_positions(c, dX, pi, pj);
-    if (rc == OD_OK) rc = launch_pair(c, pj, pj, dF, false);
+    if (rc == OD_OK) rc = launch_forces(c, dX, pj, nullptr, dF, false);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(sf.data(), dF, sf.size() * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

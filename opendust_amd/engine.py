@@ -27,6 +27,7 @@ RUN_PROFILE_PAIR = 2
 PAIR_AUTO = 0
 PAIR_ROWS = 1
 PAIR_SYM = 2
+PAIR_CELL = 3
 
 _dv = ctypes.c_double * MAX_DUST
 
@@ -58,6 +59,7 @@ class Table(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [
         ("abi_version", ctypes.c_int32),
+        ("struct_size", ctypes.c_int32),
         ("n_ions", ctypes.c_int32),
         ("n_dust", ctypes.c_int32),
         ("distribution", ctypes.c_int32),
@@ -67,6 +69,7 @@ class Config(ctypes.Structure):
         ("j_split", ctypes.c_int32),
         ("pair_mode", ctypes.c_int32),
         ("sym_chunk", ctypes.c_int32),
+        ("reserved_", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("R", ctypes.c_double), ("H", ctypes.c_double), ("dt", ctypes.c_double),
         ("r_D_e", ctypes.c_double), ("r_0", ctypes.c_double),
@@ -77,6 +80,8 @@ class Config(ctypes.Structure):
         ("trap_C", ctypes.c_double),
         ("trap_alpha", ctypes.c_double * 15),
         ("field_force", ctypes.c_double),
+        ("cutoff", ctypes.c_double),
+        ("switch_dist", ctypes.c_double),
         ("thr_top", ctypes.c_uint64),
         ("thr_side", ctypes.c_uint64),
         ("thr_coll", ctypes.c_uint64),
@@ -133,6 +138,7 @@ _lib = None
 EXPORTS = [
     "od_create", "od_set_ions", "od_get_ions", "od_set_dust_charges", "od_get_dust_charges",
     "od_set_step", "od_run", "od_pair_timing", "od_pair_info", "od_get_step_info", "od_pair_forces",
+    "od_get_cells", "od_cutoff_pair_count",
     "od_trap_potential_grid", "od_trap_samples_host", "od_abi_sizes", "od_destroy",
     "od_last_error", "od_build_info",
 ]
@@ -160,6 +166,8 @@ def load_library(path=LIB_PATH):
     lib.od_run.argtypes = [c_ctx, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32]
     lib.od_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), ctypes.c_int32]
     lib.od_pair_info.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int64)]
+    lib.od_get_cells.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_int32)]
+    lib.od_cutoff_pair_count.argtypes = [c_ctx, P(ctypes.c_int64), P(ctypes.c_int64)]
     lib.od_get_step_info.argtypes = [c_ctx, P(ctypes.c_uint32), P(ctypes.c_uint8), P(ctypes.c_int64)]
     lib.od_pair_forces.argtypes = [c_ctx, dp, dp]
     lib.od_trap_potential_grid.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, dp,
@@ -220,6 +228,7 @@ class Engine:
 
     def __init__(self, cfg):
         self.lib = load_library()
+        cfg.struct_size = ctypes.sizeof(Config)
         self.cfg = cfg
         self.n = cfg.n_ions
         self.n_dust = cfg.n_dust
@@ -280,6 +289,23 @@ class Engine:
         pairs = ctypes.c_int64()
         _check(self.lib.od_pair_info(self._ctx, ctypes.byref(mode), ctypes.byref(pairs)), "od_pair_info")
         return mode.value, pairs.value
+
+    def cells(self):
+        """(cell_of_ion[n], sorted_ions[n_binned], dims) of the last cell list (cutoff mode)."""
+        cell = np.empty(self.n, dtype=np.int32)
+        order = np.empty(self.n, dtype=np.int32)
+        nb = ctypes.c_int64()
+        dims = (ctypes.c_int32 * 3)()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _check(self.lib.od_get_cells(self._ctx, cell.ctypes.data_as(i32), order.ctypes.data_as(i32),
+                                     ctypes.byref(nb), dims), "od_get_cells")
+        return cell, order[: nb.value], tuple(dims)
+
+    def cutoff_pair_count(self):
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        _check(self.lib.od_cutoff_pair_count(self._ctx, ctypes.byref(a), ctypes.byref(b)), "od_cutoff_pair_count")
+        return a.value, b.value
 
     def step_info(self):
         flags = np.empty(self.n, dtype=np.uint32)

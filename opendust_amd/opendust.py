@@ -12,6 +12,7 @@ engine (opendust_amd/csrc/od_engine.hip) driven through the C ABI.
 """
 
 import csv
+import ctypes
 import secrets
 
 import numpy as np
@@ -192,10 +193,15 @@ class OpenDust:
       seed       Philox key of the in-step randomness (None: fresh random key)
       useGraph   replay steps as captured HIP graphs
       trapSeed, trapSamples   Monte-Carlo seed / sample count of the trap fit
+      cutoffDistance, switchingDistance   (m) truncation used when
+                 simulate(cutOff=True); default 3.2 and 3 r_D_e as the reference
+                 (opendust.py:481-485); switchingDistance=None with a given
+                 cutoffDistance means a hard cutoff
     """
 
     def __init__(self, plasmaParametersInSIUnits, simulationParametersInSIUnits, outputParameters, dustParticles,
-                 distributionType, precision="fp32c", seed=None, useGraph=True, trapSeed=None, trapSamples=None):
+                 distributionType, precision="fp32c", seed=None, useGraph=True, trapSeed=None, trapSamples=None,
+                 cutoffDistance=None, switchingDistance=None):
         self.plasmaParametersInSIUnits = plasmaParametersInSIUnits
         self.simulationParametersInSIUnits = simulationParametersInSIUnits
         self.outputParameters = outputParameters
@@ -240,6 +246,10 @@ class OpenDust:
         self.useGraph = useGraph
         self.trapSeed = trapSeed
         self.trapSamples = trapSamples
+        self.cutoffDistance = cutoffDistance
+        self.switchingDistance = switchingDistance
+        self._cutoff_nm = 0.0
+        self._switch_nm = 0.0
         self.alphaTrap = None
         self.p_t = self.p_s = self.p_b = None
 
@@ -276,6 +286,18 @@ class OpenDust:
         pot = trap_potential_grid(Rl, Hl, rho, h, n_mc, seed, device)
         return host.trap_fit(pot, Rl, Hl, self.N)
 
+    def set_cutoff(self, cutOff):
+        """Truncation of the ion-ion force (nm) for a run; cutOff=False: all pairs."""
+        if not cutOff:
+            self._cutoff_nm = self._switch_nm = 0.0
+        elif self.cutoffDistance is None:
+            self._cutoff_nm = 3.2 * self.r_D_e
+            self._switch_nm = 3.0 * self.r_D_e
+        else:
+            self._cutoff_nm = float(self.cutoffDistance * host.M_TO_NM)
+            self._switch_nm = (float(self.switchingDistance * host.M_TO_NM)
+                               if self.switchingDistance is not None else 0.0)
+
     def make_config(self, tables, alphaTrap, device=0, precision=None, seed=None):
         """Fill the engine configuration (include/opendust_amd.h od_config)."""
         pp, sp = self.plasmaParametersInSIUnits, self.simulationParametersInSIUnits
@@ -283,6 +305,7 @@ class OpenDust:
         prec = precision or self.precision
         c = Config()
         c.abi_version = ABI_VERSION
+        c.struct_size = ctypes.sizeof(Config)
         c.n_ions = int(self.N)
         c.n_dust = self.numberOfDustParticles
         c.distribution = host.DIST_FIELD_DRIVEN if self.distributionType == "fieldDriven" else host.DIST_MAXWELLIAN
@@ -308,6 +331,8 @@ class OpenDust:
         c.trap_C = self.q_s * self.q_s / pp.r_D_e * self.k * host.J_TO_KJ_PER_MOL
         for m in range(15):
             c.trap_alpha[m] = float(alphaTrap[m])
+        c.cutoff = self._cutoff_nm
+        c.switch_dist = self._switch_nm
         if self.distributionType == "fieldDriven":
             c.field_force = self.E * self.externalFieldConversionCoefficient * self.q_s
             c.thr_coll = host.u32_threshold(self.d_t * self.w_c)
@@ -349,9 +374,7 @@ class OpenDust:
     # ------------------------------------------------------------ simulation
     def simulate(self, deviceIndex="0", cutOff=False, toRestartFileName=""):
         """Run the simulation (opendust.py:362-1384)."""
-        if cutOff:
-            raise NotImplementedError(
-                "cutOff=True (OpenMM CutoffNonPeriodic 3.2 r_D_e with switch) is not available in this engine")
+        self.set_cutoff(cutOff)
         device = 0
         dev_str = str(deviceIndex).strip()
         if dev_str:

@@ -9,6 +9,8 @@ and initial state without running ``simulate()``.
   dust_chain      field-driven flow with charge-exchange collisions,
                   w_c = 0.1 w_pi, E such that M = 0.8, three dusts on the axis
                   at z = -1, 0, +1 r_D_e, R = 6, H = 14 r_D_e, N = 2^20.
+  cutoff_large    single dust, N = 2^22, R = 10, H = 20 r_D_e, ion-ion force
+                  truncated at 2 r_D_e through the cell list.
   piel_fig1       the tutorial / Piel 2017 Fig. 1 setting (n = 1e14 m^-3,
                   R = 3, H = 6 r_D_e, r = 58.8 um at z = -r_D_e, N = 2^16).
 """
@@ -32,6 +34,19 @@ def single_dust(N=16384, n=2000, precision="fp32c", seed=1, charge="consistent",
     dust = [DustParticle(0.0, 0.0, 0.0, 0.05 * lam, charge, -2e4 * abs(pp.e))]
     return OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dust, "Maxwellian", precision=precision, seed=seed,
                     **kw)
+
+
+def cutoff_large(N=2 ** 22, n=10000, precision="fp32c", seed=1, cutoff=2.0, switch=None, **kw):
+    """Single dust as single_dust, N = 2^22 in a cylinder R = 10, H = 20 r_D_e,
+    ion-ion force truncated at `cutoff` r_D_e (hard, or switched from `switch`)."""
+    pp = PlasmaParametersInSIUnitsMaxwell(T_I, T_E, N_INF, 1.0, M_AR)
+    lam = pp.r_D_e
+    sp = SimulationParametersInSIUnits(10 * lam, 20 * lam, int(N), int(n), 0.01 / pp.w_p_i, pp)
+    dust = [DustParticle(0.0, 0.0, 0.0, 0.05 * lam, "consistent", -2e4 * abs(pp.e))]
+    od = OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dust, "Maxwellian", precision=precision, seed=seed,
+                  cutoffDistance=cutoff * lam, switchingDistance=None if switch is None else switch * lam, **kw)
+    od.set_cutoff(True)
+    return od
 
 
 def dust_chain(N=2 ** 20, n=20000, precision="fp32c", seed=1, **kw):

@@ -98,9 +98,15 @@ double orc_table_eval(const od_table* t, double u) {
 static const int MONO[15][2] = {{0, 0}, {2, 0}, {0, 2}, {2, 2}, {4, 0}, {0, 4}, {4, 2}, {2, 4},
                                 {6, 0}, {0, 6}, {4, 4}, {6, 2}, {2, 6}, {8, 0}, {0, 8}};
 
-/* ion-ion force on rows [i0, i1) (kJ/mol/nm) */
+/* ion-ion force on rows [i0, i1) (kJ/mol/nm).  With cfg->cutoff > 0 only
+ * pairs with r < cutoff interact; with 0 < switch_dist < cutoff the energy is
+ * U(r) S(x), S = 1 - 10x^3 + 15x^4 - 6x^5, x = (r - switch)/(cutoff - switch)
+ * (OpenMM's switching function, opendust.py:483-485), and the force is
+ * -d(U S)/dr. */
 void orc_pair_forces(const od_config* c, int n, const double* x, double* f, int i0, int i1, int nthreads) {
   const double lam = c->r_D_e, r0 = c->r_0, kqq = c->kqq;
+  const int cut = c->cutoff > 0.0;
+  const int sw = cut && c->switch_dist > 0.0 && c->switch_dist < c->cutoff;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
@@ -111,9 +117,18 @@ void orc_pair_forces(const od_config* c, int n, const double* x, double* f, int 
       if (j == i) continue;
       const double dx = xi - x[3 * j], dy = yi - x[3 * j + 1], dz = zi - x[3 * j + 2];
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (cut && !(r < c->cutoff)) continue;
       const double rr = r + r0;
       /* -dU/dr for U = kqq e^{-r/lambda} / (r + r_0) */
-      const double mag = kqq * exp(-r / lam) * (1.0 / (lam * rr) + 1.0 / (rr * rr));
+      double mag = kqq * exp(-r / lam) * (1.0 / (lam * rr) + 1.0 / (rr * rr));
+      if (sw && r > c->switch_dist) {
+        const double width = c->cutoff - c->switch_dist;
+        const double t = (r - c->switch_dist) / width;
+        const double S = 1.0 - 10.0 * t * t * t + 15.0 * t * t * t * t - 6.0 * t * t * t * t * t;
+        const double dSdr = (-30.0 * t * t + 60.0 * t * t * t - 30.0 * t * t * t * t) / width;
+        const double U = kqq * exp(-r / lam) / rr;
+        mag = mag * S - U * dSdr;
+      }
       fx += mag * dx / r;
       fy += mag * dy / r;
       fz += mag * dz / r;
@@ -416,3 +431,19 @@ int orc_max_threads(void) {
 
 /* layout check against the ctypes mirror */
 int64_t orc_config_size(void) { return (int64_t)sizeof(od_config); }
+
+/* cell of a position under the engine's binning specification (opendust_amd.h) */
+int orc_cell_of(const od_config* c, const double* x, int* dims) {
+  const double h = c->cutoff / 2.0;
+  const int nx = (int)ceil(2.0 * c->R / h), ny = nx, nz = (int)ceil(c->H / h);
+  int ix = (int)floor((x[0] + c->R) / h), iy = (int)floor((x[1] + c->R) / h), iz = (int)floor((x[2] + c->H / 2.0) / h);
+  ix = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);
+  iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
+  iz = iz < 0 ? 0 : (iz >= nz ? nz - 1 : iz);
+  if (dims) {
+    dims[0] = nx;
+    dims[1] = ny;
+    dims[2] = nz;
+  }
+  return ix + nx * (iy + ny * iz);
+}

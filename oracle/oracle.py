@@ -60,6 +60,8 @@ def load():
                                   ctypes.c_int64, ctypes.c_uint64, dp, ctypes.c_int]
     lib.orc_max_threads.restype = ctypes.c_int
     lib.orc_config_size.restype = ctypes.c_int64
+    lib.orc_cell_of.argtypes = [P(Config), dp, P(ctypes.c_int)]
+    lib.orc_cell_of.restype = ctypes.c_int
     if lib.orc_config_size() != ctypes.sizeof(Config):
         raise RuntimeError("oracle library was built against a different od_config layout; run make -C oracle")
     _lib = lib
@@ -194,3 +196,11 @@ def trap_grid(R, H, rho, h, n_mc, seed, nthreads=0):
     if rc:
         raise RuntimeError("orc_trap_grid failed")
     return out
+
+
+def cell_of(cfg, x):
+    """(cell id, (nx, ny, nz)) of a position under the engine's binning."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dims = (ctypes.c_int * 3)()
+    cid = load().orc_cell_of(ctypes.byref(cfg), _d(x), dims)
+    return cid, tuple(dims)

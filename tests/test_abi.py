@@ -49,6 +49,7 @@ def test_struct_sizes_match(eng):
 def test_invalid_config_is_an_error_not_a_crash(eng):
     cfg = eng.Config()
     cfg.abi_version = eng.ABI_VERSION
+    cfg.struct_size = ctypes.sizeof(eng.Config)
     cfg.n_ions = 0
     ctx = ctypes.c_void_p()
     rc = eng.load_library().od_create(ctypes.byref(cfg), ctypes.byref(ctx))
@@ -60,6 +61,12 @@ def test_invalid_config_is_an_error_not_a_crash(eng):
     assert rc == 1 and b"ABI" in eng.load_library().od_last_error()
     with pytest.raises(RuntimeError):
         eng.Engine(cfg)
+    # a caller with another layout of od_config is refused before anything runs
+    cfg.abi_version = eng.ABI_VERSION
+    cfg.n_ions = 16
+    cfg.struct_size = ctypes.sizeof(eng.Config) - 8
+    rc = eng.load_library().od_create(ctypes.byref(cfg), ctypes.byref(ctx))
+    assert rc == 1 and b"layout" in eng.load_library().od_last_error()
 
 
 def test_engine_host_sampler_matches_oracle(eng):

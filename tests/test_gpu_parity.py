@@ -88,12 +88,14 @@ def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
-@pytest.mark.parametrize("precision,mode", [("fp64", "rows"), ("fp32c", "rows"), ("fp32c", "sym")])
-def test_one_step_vs_oracle(oracle_lib, precision, mode):
+@pytest.mark.parametrize("precision,mode,workload", [
+    ("fp64", "rows", "single_dust"), ("fp32c", "rows", "single_dust"), ("fp32c", "sym", "single_dust"),
+    ("fp64", "rows", "dust_chain"), ("fp32c", "sym", "dust_chain")])
+def test_one_step_vs_oracle(oracle_lib, precision, mode, workload):
     eng = _engine_mod()
-    od, cfg, x0, v0 = _setup(N=2048, precision=precision, mode=mode)
+    od, cfg, x0, v0 = _setup(N=2048, precision=precision, mode=mode, workload=workload)
     q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
-    say("one step", precision, mode, "N", cfg.n_ions)
+    say("one step", precision, mode, workload, "N", cfg.n_ions)
     e = eng.Engine(cfg)
     e.set_ions(x0, v0)
     e.set_step(5)
@@ -120,11 +122,12 @@ def test_one_step_vs_oracle(oracle_lib, precision, mode):
         assert np.max(np.abs(a - b)) <= tol_r * sc, key
 
 
-def test_multi_step_fp64_records(oracle_lib):
+@pytest.mark.parametrize("workload", ["single_dust", "dust_chain"])
+def test_multi_step_fp64_records(oracle_lib, workload):
     eng = _engine_mod()
-    od, cfg, x0, v0 = _setup(N=1024, precision="fp64", craft=True)
+    od, cfg, x0, v0 = _setup(N=1024, precision="fp64", craft=True, workload=workload)
     q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
-    say("multi step fp64 N", cfg.n_ions)
+    say("multi step fp64", workload, "N", cfg.n_ions)
     e = eng.Engine(cfg)
     e.set_ions(x0, v0)
     rec = e.run(6)
@@ -156,6 +159,76 @@ def test_determinism_and_graph(oracle_lib, mode):
         assert np.array_equal(rec, outs[0][0])
         assert np.array_equal(x, outs[0][1])
         assert np.array_equal(v, outs[0][2])
+
+
+CUT_CASES = [("fp64", 1.5, None), ("fp32c", 1.5, None), ("fp32c", 3.2, 3.0)]
+
+
+@pytest.mark.parametrize("precision,rc,rs", CUT_CASES)
+def test_cutoff_pair_forces_and_cells(oracle_lib, precision, rc, rs):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=3000, precision=precision, craft=False)
+    cfg.cutoff = rc * cfg.r_D_e
+    cfg.switch_dist = 0.0 if rs is None else rs * cfg.r_D_e
+    say("cutoff pair forces", precision, rc, rs, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    assert e.pair_info()[0] == eng.PAIR_CELL
+    f_gpu = e.pair_forces(x0)
+    cell, order, dims = e.cells()
+    e.close()
+    f_orc = oracle_lib.pair_forces(cfg, x0)
+    fn = np.linalg.norm(f_orc, axis=1)
+    err = np.linalg.norm(f_gpu - f_orc, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    say("  norm-wise rel", np.linalg.norm(f_gpu - f_orc) / np.linalg.norm(f_orc))
+    if precision == "fp64":
+        assert np.all(err <= 1e-11 * fn + 1e-13 * rms)
+    else:
+        assert np.linalg.norm(f_gpu - f_orc) <= 1e-5 * np.linalg.norm(f_orc)
+        assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
+    # binning is bit-exact, the order is (cell, ion index)
+    ref = np.array([oracle_lib.cell_of(cfg, xi)[0] for xi in x0])
+    assert dims == oracle_lib.cell_of(cfg, x0[0])[1]
+    assert np.array_equal(cell, ref)
+    assert np.array_equal(order, np.lexsort((np.arange(len(ref)), ref)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_cutoff_one_step_and_determinism(oracle_lib, precision):
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=2048, precision=precision, craft=True)
+    cfg.cutoff = 1.5 * cfg.r_D_e
+    q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
+    say("cutoff one step", precision, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    e.set_step(3)
+    rec = e.run(1)
+    x_g, v_g = e.get_ions()
+    fl_g, face_g, nre = e.step_info()
+    cell, order, dims = e.cells()
+    e.close()
+    x_o, v_o, q_o, rec_o, fl_o, face_o = oracle_lib.step(cfg, x0, v0, q0, 3)
+    assert np.array_equal(fl_g, fl_o) and np.array_equal(face_g, face_o) and nre >= 18
+    assert np.array_equal(rec["absorbed"][0], rec_o["absorbed"])
+    # the cell list of the step holds the kept ions at their drifted positions
+    kept = fl_o == 0
+    assert len(order) == int(kept.sum())
+    assert np.max(np.abs(x_g - x_o)) <= (1e-11 if precision == "fp64" else 1e-8) * cfg.R
+    tol_r = 1e-9 if precision == "fp64" else 1e-5
+    for key in ("q", "force_orbit", "force_collect"):
+        sc = max(np.max(np.abs(rec_o[key])), 1e-300)
+        assert np.max(np.abs(rec[key][0] - rec_o[key])) <= tol_r * sc, key
+    # determinism: eager twice and graph replay, bitwise
+    outs = []
+    for flags in (0, 0, eng.RUN_GRAPH):
+        e = eng.Engine(cfg)
+        e.set_ions(x0, v0)
+        r = e.run(12, flags)
+        outs.append((r, e.get_ions()[0]))
+        e.close()
+    for r, x in outs[1:]:
+        assert np.array_equal(r, outs[0][0]) and np.array_equal(x, outs[0][1])
 
 
 def test_trap_grid_vs_oracle(oracle_lib):

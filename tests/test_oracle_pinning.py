@@ -93,6 +93,63 @@ def test_pair_force_is_minus_gradient_of_reference_energy(orc, base):
     assert np.linalg.norm(f.sum(0)) <= 1e-12 * np.abs(f).sum()
 
 
+def _switched_energy(cfg, x, rc, rs):
+    e = 0.0
+    for i in range(len(x)):
+        d = x[i + 1:] - x[i]
+        r = np.sqrt((d ** 2).sum(1))
+        u = cfg.kqq * np.exp(-r / cfg.r_D_e) / (r + cfg.r_0)
+        t = np.clip((r - rs) / (rc - rs), 0.0, 1.0)
+        S = 1 - 10 * t ** 3 + 15 * t ** 4 - 6 * t ** 5
+        e += np.sum(np.where(r < rc, u * S, 0.0))
+    return e
+
+
+@pytest.mark.parametrize("switch", [True, False])
+def test_truncated_force_is_minus_gradient_of_switched_energy(orc, base, switch):
+    """OpenMM's switched cutoff (opendust.py:481-485): U S(x), S = 1 - 10x^3 +
+    15x^4 - 6x^5 between the switching distance and the cutoff."""
+    _, cfg, _, _ = base
+    lam = cfg.r_D_e
+    rng = np.random.RandomState(4)
+    x = rng.uniform(-2.0, 2.0, size=(24, 3)) * lam
+    x[1] = x[0] + np.array([3.1, 0.0, 0.0]) * lam  # inside the switching band
+    rc, rs = 3.2 * lam, (3.0 * lam if switch else 3.2 * lam)
+    cfg.cutoff, cfg.switch_dist = rc, (rs if switch else 0.0)
+    try:
+        f = orc.pair_forces(cfg, x, nthreads=2)
+        h = 1e-6 * lam
+        for i in (0, 1, 5):
+            for k in range(3):
+                xp = x.copy()
+                xm = x.copy()
+                xp[i, k] += h
+                xm[i, k] -= h
+                fd = -(_switched_energy(cfg, xp, rc, rs) - _switched_energy(cfg, xm, rc, rs)) / (2 * h)
+                assert abs(fd - f[i, k]) <= 1e-6 * np.abs(f[i]).max() + 1e-12 * np.abs(f).max()
+        # pairs beyond the cutoff do not interact
+        far = np.array([[0.0, 0.0, 0.0], [3.3 * lam, 0.0, 0.0]])
+        assert np.all(orc.pair_forces(cfg, far, nthreads=1) == 0.0)
+    finally:
+        cfg.cutoff = cfg.switch_dist = 0.0
+
+
+def test_binning_spec(orc, base):
+    _, cfg, _, _ = base
+    cfg.cutoff = 1.5 * cfg.r_D_e
+    try:
+        h = cfg.cutoff / 2
+        cid, dims = orc.cell_of(cfg, np.array([-cfg.R, -cfg.R, -cfg.H / 2]))
+        assert cid == 0 and dims == (int(np.ceil(2 * cfg.R / h)),) * 2 + (int(np.ceil(cfg.H / h)),)
+        nx, ny, nz = dims
+        cid, _ = orc.cell_of(cfg, np.array([cfg.R * 2, cfg.R * 2, cfg.H]))  # clamped to the last cell
+        assert cid == nx * ny * nz - 1
+        p = np.array([-cfg.R + 2.5 * h, -cfg.R + 0.5 * h, -cfg.H / 2 + 1.5 * h])
+        assert orc.cell_of(cfg, p)[0] == 2 + nx * (0 + ny * 1)
+    finally:
+        cfg.cutoff = 0.0
+
+
 def _ext_energy(cfg, q, x):
     e = 0.0
     for p in range(cfg.n_dust):
