@@ -29,7 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector with packed ops; also matrix), TFLOP/s
-FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term (see DESIGN.md)
+FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
+# HBM bytes per k_pair32_sym launch at 2^20 ions (default source chunk), from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy's readings (x2.000,
+# x1.000): profiles/k_pair32_sym_2e20_pmc.md
+SYM_TRAFFIC_2E20 = 13.405e9
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
 
 
@@ -108,6 +112,65 @@ def cutoff_leg(steps):
     }
 
 
+def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
+    """Parameter sweep: 64 independent simulations (N log-uniform in [2^12,
+    2^17], M in [0.5, 2], 1-4 dusts), FP32c, graph-replayed steps.  Batched
+    (one od_batch) against the same simulations run one after another through
+    single contexts; the step loop of a sweep_len-step sweep is extrapolated
+    from `steps` (batched) and `seq_steps` (per simulation) timed steps.  Setup
+    (initial states, trap fit) is not timed on either side."""
+    import contextlib
+    import io
+
+    from opendust_amd import engine, sweep, workloads
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        ods = workloads.sweep(n_sims=n_sims, n=sweep_len, precision="fp32c", seed=7, trapSeed=5,
+                              trapSamples=4_000_000)
+    np.random.seed(21)
+    cfgs, states = sweep.prepare_batch(ods, device=0)
+    pairs = float(sum(c.n_ions * (c.n_ions - 1.0) for c in cfgs))
+    ions = int(sum(c.n_ions for c in cfgs))
+    b = engine.Batch(cfgs)
+    for s_, (x0, v0) in enumerate(states):
+        b.set_ions(s_, x0, v0)
+    b.run(3, engine.RUN_PROFILE_PAIR, records=False)
+    b.pair_timing(reset=True)
+    b.run(3, engine.RUN_PROFILE_PAIR, records=False)
+    kms, nl, ppl = b.pair_timing()
+    b.run(16, engine.RUN_GRAPH, records=False)
+    t0 = time.perf_counter()
+    b.run(steps, engine.RUN_GRAPH, records=False)
+    t_batch = (time.perf_counter() - t0) / steps
+    b.close()
+    log("sweep batched: %.2f ms/step" % (t_batch * 1e3))
+    t_seq = 0.0
+    for s_, (cfg, (x0, v0)) in enumerate(zip(cfgs, states)):
+        e = engine.Engine(cfg)
+        e.set_ions(x0, v0)
+        e.run(seq_steps, engine.RUN_GRAPH)  # captures the graph the timed run replays
+        t0 = time.perf_counter()
+        e.run(seq_steps, engine.RUN_GRAPH)
+        t_seq += (time.perf_counter() - t0) / seq_steps
+        e.close()
+    log("sweep sequential: %.2f ms per step of all simulations" % (t_seq * 1e3))
+    kernel_ms = kms / max(nl, 1)
+    return {
+        "workload": "%d simulations, N log-uniform in [2^12, 2^17] (sum %d ions), M uniform in [0.5, 2], 1-4 dusts "
+                    "on the axis, FP32c, %d steps each" % (n_sims, ions, sweep_len),
+        "pairs_per_step": pairs,
+        "batched_ms_per_step": t_batch * 1e3,
+        "sequential_ms_per_step": t_seq * 1e3,
+        "sims_per_hour_batched": n_sims / (sweep_len * t_batch / 3600.0),
+        "sims_per_hour_sequential": n_sims / (sweep_len * t_seq / 3600.0),
+        "batched_over_sequential": t_seq / t_batch,
+        "pair_interactions_per_s_batched": pairs / t_batch,
+        "batch_pair_kernel_ms": kernel_ms,
+        "batch_pair_kernel_tflops_alg": ppl * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12,
+        "timed_steps": {"batched": steps, "sequential_per_sim": seq_steps},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +184,9 @@ def main():
     ap.add_argument("--pair-mode", choices=["sym", "rows", "auto"], default="sym")
     ap.add_argument("--cutoff-steps", type=int, default=0,
                     help="also time the cutoff workload (2^22 ions, R=10, H=20 r_D_e, r_c=2 r_D_e)")
+    ap.add_argument("--sweep-steps", type=int, default=0,
+                    help="also time the 64-simulation sweep, batched (this many steps) and sequential")
+    ap.add_argument("--sweep-seq-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -206,6 +272,12 @@ def main():
         cut = cutoff_leg(args.cutoff_steps)
         log("cutoff:", cut)
 
+    swp = None
+    if args.sweep_steps > 0:
+        log("sweep workload")
+        swp = sweep_leg(args.sweep_steps, args.sweep_seq_steps)
+        log("sweep:", swp)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -240,7 +312,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": SYM_TRAFFIC_2E20 if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
+            "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
             "flop_per_pair": FLOP_PER_PAIR,
@@ -249,6 +322,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
         "cutoff_2e22": cut,
+        "sweep_64": swp,
         "absorbed_last_step": int(rec["absorbed"][-1, 0]),
     }
     print(json.dumps(out))

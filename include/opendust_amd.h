@@ -84,7 +84,7 @@ typedef struct od_config {
   int32_t j_split;       /* source chunks of the row pair kernel; 0 = automatic */
   int32_t pair_mode;     /* OD_PAIR_*: which all-pairs kernel */
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
-  int32_t reserved_;
+  int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
   uint64_t seed;         /* Philox4x32-10 key */
 
   /* geometry, integration (OpenMM units) */
@@ -222,6 +222,43 @@ int od_trap_potential_grid(int32_t device, double R, double H, int32_t n_r, cons
 /* Sample coordinates of the Monte Carlo above (for checking the sampler):
  * k in [k0, k0 + n) -> xyz[n][3], units of r_D_e.  Pure host computation. */
 int od_trap_samples_host(double R, double H, uint64_t seed, int64_t k0, int64_t n, double* xyz);
+
+/* ---- batched parameter sweeps ------------------------------------------
+ * Many independent simulations advanced together on one device: one launch of
+ * each step kernel covers all of them, and the all-pairs work of all
+ * simulations is a single launch over (simulation, target block, source chunk)
+ * items.  Replaces a sequential Python loop of OpenDust(...).simulate() calls
+ * (examples/Piel2017/Figure5/launchPiel2017Figure5.py:30); the reference has no
+ * batch interface.  Every simulation has its own od_config (any ion and dust
+ * count, plasma and geometry); all share precision and device, and run
+ * all-pairs forces.  cfg.sim_id is the simulation's Philox counter word, so a
+ * batched simulation draws the same random numbers as an od_ctx created with
+ * the same od_config. */
+typedef struct od_batch od_batch;
+
+int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch** out);
+
+/* Initial ions of simulation sim: x, v [n][3] (n = its cfg.n_ions). */
+int od_batch_set_ions(od_batch* b, int32_t sim, const double* x, const double* v, int64_t n);
+int od_batch_get_ions(od_batch* b, int32_t sim, double* x, double* v);
+
+/* nsteps steps of every simulation.  out: caller-owned
+ * [nsteps][n_sims][OD_MAX_DUST] records (slots p >= a simulation's n_dust are
+ * zero; may be NULL).  flags: OD_RUN_GRAPH, OD_RUN_PROFILE_PAIR. */
+int od_batch_run(od_batch* b, int64_t nsteps, od_dust_record* out, int32_t flags);
+
+/* Flags and faces of simulation sim's ions in the last step (as od_get_step_info). */
+int od_batch_get_step_info(od_batch* b, int32_t sim, uint32_t* flags, uint8_t* face);
+
+/* HIP-event time of the batched pair launches timed with OD_RUN_PROFILE_PAIR
+ * since the last reset, their count, and the ordered pairs of real ions one
+ * launch covers: sum over simulations of n(n-1), less the pairs inside one
+ * 1024-ion block when the symmetric FP32 kernel runs (fp32c and pair_mode !=
+ * OD_PAIR_ROWS in the first configuration).  Measurement support. */
+int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* launches, int64_t* pairs_per_launch,
+                         int32_t reset);
+
+int od_batch_destroy(od_batch* b);
 
 /* sizeof(od_config), sizeof(od_dust_record) as compiled (ABI check, no GPU use). */
 int od_abi_sizes(int64_t* config_size, int64_t* record_size);

@@ -68,6 +68,12 @@ static int set_err(int code, const char* fmt, ...) {
 struct DevParams {
   ExtParams ext;
   int n, n_pad, n_dust, distribution, use_bottom;
+  // position of this simulation in the (possibly batched) arrays
+  int ion_off;   // first global ion index
+  int stride;    // SoA stride of X, V, F (all simulations)
+  int blk0, blk1;  // its 256-ion blocks [blk0, blk1)
+  int q_off;     // its dust charges in q[]
+  uint32_t sim_id;  // Philox counter word 2
   uint32_t dust_mask;  // flag bits of the dusts
   uint64_t seed;
   double R, H, half_H, dt, half_dt, m_s, v_T, v_fl, lam;
@@ -147,6 +153,11 @@ __device__ __forceinline__ void pair_acc<double>(const DevParams& P, const doubl
   else
     pair64_cut(i.x, i.y, i.z, j.x, j.y, j.z, P.rho0, P.use_switch != 0, P.rc64, P.rs64, P.invw64, ax.a, ay.a,
                az.a);
+}
+
+// parameters of the simulation a 256-ion block belongs to (blk_sim == null: one simulation)
+__device__ __forceinline__ const DevParams* sim_of_block(const DevParams* Pall, const int* blk_sim, int blk) {
+  return blk_sim ? Pall + blk_sim[blk] : Pall;
 }
 
 // deterministic block sum of a double over a 256-thread block (value valid on all threads)
@@ -371,15 +382,19 @@ __device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& 
   bz = __shfl_xor(bz, mask, 64);
 }
 
-// tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials
+// tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials.
+// Batched simulations: blk_end[I] (first block past I's simulation) and
+// blk_a[I] (its a) per block; null for one simulation (n_blocks, a).
 __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
                                                     const int2* __restrict__ wg_tab, float a, float linv,
+                                                    const int* __restrict__ blk_end, const float* __restrict__ blk_a,
                                                     double* __restrict__ tp, float* __restrict__ sp) {
   __shared__ double red[4][3][64];
   const int2 ik = wg_tab[blockIdx.x];
   const int I = ik.x;
   const int J0 = I + 1 + ik.y * ch;
-  const int J1 = min(J0 + ch, n_blocks);
+  const int J1 = min(J0 + ch, blk_end ? blk_end[I] : n_blocks);
+  if (blk_a) a = blk_a[I];
   const int nsub = (J1 - J0) * (SYM_TB / 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t sp_len = (size_t)ch * SYM_TB;
@@ -441,8 +456,10 @@ __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ p
 
 // pairs inside one block (ordered, sources via scalar loads); dg: [3][n_pad]
 __global__ __launch_bounds__(256) void k_pair32_diag(const float4* __restrict__ pos, int n_pad, float a,
-                                                     float linv, double* __restrict__ dg) {
+                                                     float linv, const float* __restrict__ blk_a,
+                                                     double* __restrict__ dg) {
   const int X = blockIdx.x / (SYM_TB / 256), q = blockIdx.x % (SYM_TB / 256);
+  if (blk_a) a = blk_a[X];
   const int i = X * SYM_TB + q * 256 + threadIdx.x;
   const float4 pi = pos[i];
   float ax = 0.f, ay = 0.f, az = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
@@ -463,20 +480,25 @@ __global__ __launch_bounds__(256) void k_pair32_diag(const float4* __restrict__ 
   dg[2 * n_pad + i] = (double)sz - (double)cz;
 }
 
-// F = scale * (diagonal + target partials + source partials), fixed order
+// F = scale * (diagonal + target partials + source partials), fixed order.
+// Batched simulations: blk_begin[X] (first block of X's simulation) and
+// blk_scale[X] per block; null for one simulation (0, scale).
 __global__ __launch_bounds__(256) void k_sym_reduce(const double* __restrict__ dg, const double* __restrict__ tp,
                                                     const float* __restrict__ sp, const int* __restrict__ wg_base,
                                                     int n_blocks, int ch, int n_pad, double scale,
-                                                    double* __restrict__ F) {
+                                                    const int* __restrict__ blk_begin,
+                                                    const double* __restrict__ blk_scale, double* __restrict__ F) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_pad) return;
   const int X = i / SYM_TB, off = i % SYM_TB;
+  const int Jb = blk_begin ? blk_begin[X] : 0;
+  if (blk_scale) scale = blk_scale[X];
   const size_t sp_len = (size_t)ch * SYM_TB;
   const int k0 = wg_base[X], k1 = wg_base[X + 1];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double s = dg[(size_t)c * n_pad + i];
-    for (int J = 0; J < X; ++J) {
+    for (int J = Jb; J < X; ++J) {
       const int d = X - J - 1;
       const size_t wg = (size_t)wg_base[J] + d / ch;
       s += (double)sp[(wg * 3 + c) * sp_len + (size_t)(d % ch) * SYM_TB + off];
@@ -496,26 +518,30 @@ constexpr uint32_t FLAG_OUT_Z = 1u << 17;
 // (opendust.py:864-910).  pj gets the drifted position, or the parking spot
 // for flagged ions, which are excluded as sources of the main pair kernel.
 template <class T>
-__global__ __launch_bounds__(256) void k_kick_drift(const DevParams* __restrict__ P, double* __restrict__ X,
-                                                    double* __restrict__ V, const double* __restrict__ F,
-                                                    const double* __restrict__ q, const uint32_t* __restrict__ st,
+__global__ __launch_bounds__(256) void k_kick_drift(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
+                                                    double* __restrict__ X, double* __restrict__ V,
+                                                    const double* __restrict__ F, const double* __restrict__ qall,
+                                                    const uint32_t* __restrict__ st,
                                                     typename Pos<T>::V* __restrict__ pi,
                                                     typename Pos<T>::V* __restrict__ pj, uint32_t* __restrict__ flags,
                                                     int* __restrict__ block_cnt, int* __restrict__ err) {
-  const int n = P->n, np = P->n_pad;
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
+  const double* q = qall + P->q_off;
+  const int np = P->stride;
+  const int i = blockIdx.x * 256 + threadIdx.x;  // global index
+  const int li = i - P->ion_off;                 // index within the simulation
   int flagged = 0;
-  if (i < n) {
+  if (li < P->n) {
     double x = X[i], y = X[np + i], z = X[2 * np + i];
     double vx = V[i], vy = V[np + i], vz = V[2 * np + i];
     const uint32_t step = st[0];
     if (P->distribution == OD_DIST_FIELD_DRIVEN) {
-      const u32x4 r = philox4x32_10((uint32_t)i, step, 0u, S_COLL, P->seed);
+      const u32x4 r = philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL, P->seed);
       const bool c0 = r.v[0] < P->thr_coll, c1 = r.v[1] < P->thr_coll, c2 = r.v[2] < P->thr_coll;
       if (c0 || c1 || c2) {
         double g0, g1, g2, gd;
-        box_muller(philox4x32_10((uint32_t)i, step, 0u, S_COLL | (1u << 8), P->seed), g0, g1);
-        box_muller(philox4x32_10((uint32_t)i, step, 0u, S_COLL | (2u << 8), P->seed), g2, gd);
+        box_muller(philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL | (1u << 8), P->seed), g0, g1);
+        box_muller(philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL | (2u << 8), P->seed), g2, gd);
         if (c0) vx = P->v_T * g0;
         if (c1) vy = P->v_T * g1;
         if (c2) vz = P->v_T * g2;
@@ -613,7 +639,7 @@ __global__ __launch_bounds__(256) void k_list(const uint32_t* __restrict__ flags
 __device__ int reinject(const DevParams& P, uint32_t ion, uint32_t step, double& x, double& y, double& z, double& vx,
                         double& vy, double& vz) {
   const double two_pi = 2.0 * 3.141592653589793;
-  const uint32_t uf = philox4x32_10(ion, step, 0u, S_FACE, P.seed).v[0];
+  const uint32_t uf = philox4x32_10(ion, step, P.sim_id, S_FACE, P.seed).v[0];
   int face;
   if ((uint64_t)uf < P.thr_top)
     face = 1;
@@ -624,27 +650,27 @@ __device__ int reinject(const DevParams& P, uint32_t ion, uint32_t step, double&
   if (face == 1 || face == 3) {
     const bool top = face == 1;
     double g1, g2;
-    box_muller(philox4x32_10(ion, step, 0u, top ? S_TOPV : S_BOTV, P.seed), g1, g2);
-    const u32x4 rz = philox4x32_10(ion, step, 0u, (top ? S_TOPV : S_BOTV) | (1u << 8), P.seed);
+    box_muller(philox4x32_10(ion, step, P.sim_id, top ? S_TOPV : S_BOTV, P.seed), g1, g2);
+    const u32x4 rz = philox4x32_10(ion, step, P.sim_id, (top ? S_TOPV : S_BOTV) | (1u << 8), P.seed);
     const double uz = u53(rz.v[0], rz.v[1]);
     vx = P.v_T * g1;
     vy = P.v_T * g2;
     vz = table_eval(top ? P.top : P.bottom, uz);
-    const u32x4 rx = philox4x32_10(ion, step, 0u, top ? S_TOPX : S_BOTX, P.seed);
+    const u32x4 rx = philox4x32_10(ion, step, P.sim_id, top ? S_TOPX : S_BOTX, P.seed);
     const double ux = u53(rx.v[0], rx.v[1]), uy = u53(rx.v[2], rx.v[3]);
     const double rr = P.R * sqrt(ux);
     x = rr * cos(two_pi * uy);
     y = rr * sin(two_pi * uy);
     z = top ? -P.half_H : P.half_H;
   } else {
-    const u32x4 rx = philox4x32_10(ion, step, 0u, S_SIDEX, P.seed);
+    const u32x4 rx = philox4x32_10(ion, step, P.sim_id, S_SIDEX, P.seed);
     const double uy = u53(rx.v[0], rx.v[1]), uz = u53(rx.v[2], rx.v[3]);
     x = P.R * cos(two_pi * uy);
     y = P.R * sin(two_pi * uy);
     z = uz * P.H - P.half_H;
     double gy, gz;
-    box_muller(philox4x32_10(ion, step, 0u, S_SIDEV, P.seed), gy, gz);
-    const u32x4 ru = philox4x32_10(ion, step, 0u, S_SIDEV | (1u << 8), P.seed);
+    box_muller(philox4x32_10(ion, step, P.sim_id, S_SIDEV, P.seed), gy, gz);
+    const u32x4 ru = philox4x32_10(ion, step, P.sim_id, S_SIDEV | (1u << 8), P.seed);
     const double us = u53(ru.v[0], ru.v[1]);
     const double vs = table_eval(P.side, us);
     vx = -vs * (x / P.R) + P.v_T * gy * (y / P.R);
@@ -661,9 +687,11 @@ __device__ int reinject(const DevParams& P, uint32_t ion, uint32_t step, double&
 // momenta (opendust.py:911-921), reinjection, orbit sums at the final
 // positions (opendust.py:977-1037).  partials: [nb][OD_MAX_DUST][8].
 template <class T>
-__global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, double* __restrict__ X,
-                                               double* __restrict__ V, const double* __restrict__ F,
-                                               const double* __restrict__ q, const uint32_t* __restrict__ st,
+__global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
+                                               const int* __restrict__ block_off, int nb_all,
+                                               double* __restrict__ X, double* __restrict__ V,
+                                               const double* __restrict__ F, const double* __restrict__ qall,
+                                               const uint32_t* __restrict__ st,
                                                const typename Pos<T>::V* __restrict__ pi,
                                                typename Pos<T>::V* __restrict__ pj,
                                                const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
@@ -672,14 +700,19 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
                                                typename Pos<T>::V* __restrict__ list_new, uint8_t* __restrict__ face_out,
                                                const double* __restrict__ rowD, double* __restrict__ partials) {
   __shared__ double lds[4];
-  const int n = P->n, np = P->n_pad;
+  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
+  const double* q = qall + P->q_off;
+  const int np = P->stride;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < n;
+  const int li = i - P->ion_off;
+  const bool valid = li < P->n;
   const typename Pos<T>::V own = valid ? pi[i] : Pos<T>::park();
   Acc<T> ax, ay, az;
-  const int nf = n_flag[0];
-  for (int s0 = 0; s0 < nf; s0 += TJ) {
-    const int s1 = min(s0 + TJ, nf);
+  // this simulation's flagged ions occupy list slots [l0, l1)
+  const int l0 = block_off[P->blk0];
+  const int l1 = (P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0];
+  for (int s0 = l0; s0 < l1; s0 += TJ) {
+    const int s1 = min(s0 + TJ, l1);
     for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_old[s], ax, ay, az);
     ax.flush();
     ay.flush();
@@ -717,7 +750,7 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
     pz = vz;
     int face = 0;
     if (fl) {
-      face = reinject(*P, (uint32_t)i, st[0], x, y, z, vx, vy, vz);
+      face = reinject(*P, (uint32_t)li, st[0], x, y, z, vx, vy, vz);
       X[i] = x;
       X[np + i] = y;
       X[2 * np + i] = z;
@@ -756,20 +789,22 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ P, 
 // at the post-reinjection state (need_mask = 0), or the absorbed ions at the
 // drifted state (need_mask = dust bits: other listed ions are skipped).
 template <class T>
-__global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ P,
+__global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
                                               const typename Pos<T>::V* __restrict__ pj, const int* __restrict__ n_flag,
                                               const typename Pos<T>::V* __restrict__ list_new,
                                               const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
                                               uint32_t need_mask, double* __restrict__ rowF) {
   __shared__ double lds[4];
   const int nf = n_flag[0];
-  const int np = P->n_pad;
   for (int s = blockIdx.x; s < nf; s += gridDim.x) {
-    if (need_mask != 0u && (flags[list_ion[s]] & need_mask) == 0u) continue;
+    const int ion = list_ion[s];
+    if (need_mask != 0u && (flags[ion] & need_mask) == 0u) continue;
+    const DevParams* P = sim_of_block(Pall, blk_sim, ion >> 8);
     const typename Pos<T>::V own = list_new[s];
     Acc<T> ax, ay, az;
-    for (int j0 = threadIdx.x; j0 < np; j0 += 256 * TJ) {
-      const int j1 = min(j0 + 256 * TJ, np);
+    const int jb = P->ion_off, je = P->ion_off + P->n_pad;
+    for (int j0 = jb + (int)threadIdx.x; j0 < je; j0 += 256 * TJ) {
+      const int j1 = min(j0 + 256 * TJ, je);
       for (int j = j0; j < j1; j += 256) pair_acc<T>(*P, own, pj[j], ax, ay, az);
       ax.flush();
       ay.flush();
@@ -788,19 +823,22 @@ __global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ P,
 
 // Force of the reinjected ions on the kept ions; reinjected ions take their rows.
 template <class T>
-__global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ P, double* __restrict__ F,
+__global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
+                                                const int* __restrict__ block_off, int nb_all, double* __restrict__ F,
                                                 const typename Pos<T>::V* __restrict__ pj,
                                                 const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
                                                 const typename Pos<T>::V* __restrict__ list_new,
                                                 const int* __restrict__ slot_of, const double* __restrict__ rowF) {
-  const int n = P->n, np = P->n_pad;
+  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
+  const int np = P->stride;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < n;
+  const bool valid = i - P->ion_off < P->n;
   const typename Pos<T>::V own = valid ? pj[i] : Pos<T>::park();
   Acc<T> ax, ay, az;
-  const int nf = n_flag[0];
-  for (int s0 = 0; s0 < nf; s0 += TJ) {
-    const int s1 = min(s0 + TJ, nf);
+  const int l0 = block_off[P->blk0];
+  const int l1 = (P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0];
+  for (int s0 = l0; s0 < l1; s0 += TJ) {
+    const int s1 = min(s0 + TJ, l1);
     for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_new[s], ax, ay, az);
     ax.flush();
     ay.flush();
@@ -821,16 +859,22 @@ __global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ P,
 
 // Dust records: FP64 block-partial reduction (fixed order), charge update
 // (opendust.py:1117-1202), dust forces (opendust.py:1204-1384).
-__global__ __launch_bounds__(256) void k_dust(const DevParams* __restrict__ P, const double* __restrict__ partials,
-                                              int nb, double* __restrict__ q, uint32_t* __restrict__ st,
-                                              od_dust_record* __restrict__ rec) {
+// One block per simulation (grid = number of simulations); records go to
+// rec[(step - base) * n_sims + sim][nd_max].  The step counter is advanced by
+// block 0 when `advance` is set (one simulation), else by k_step_advance.
+__global__ __launch_bounds__(256) void k_dust(const DevParams* __restrict__ Pall, const double* __restrict__ partials,
+                                              double* __restrict__ qall, uint32_t* __restrict__ st,
+                                              od_dust_record* __restrict__ rec, int nd_max, int advance) {
   __shared__ double lds[4];
   __shared__ double tot[OD_MAX_DUST][8];
+  const DevParams* P = Pall + blockIdx.x;
+  double* q = qall + P->q_off;
   const int nd = P->n_dust;
   for (int p = 0; p < nd; ++p) {
     for (int k = 0; k < 7; ++k) {
       double s = 0.0;
-      for (int b = threadIdx.x; b < nb; b += 256) s += partials[((size_t)b * OD_MAX_DUST + p) * 8 + k];
+      for (int b = P->blk0 + (int)threadIdx.x; b < P->blk1; b += 256)
+        s += partials[((size_t)b * OD_MAX_DUST + p) * 8 + k];
       s = block_sum256(s, lds);
       if (threadIdx.x == 0) tot[p][k] = s;
     }
@@ -918,10 +962,14 @@ __global__ __launch_bounds__(256) void k_dust(const DevParams* __restrict__ P, c
     r.force_field[1] = 0.0;
     r.force_field[2] = (P->distribution == OD_DIST_FIELD_DRIVEN) ? qn[p] * P->si_E : 0.0;
     r.absorbed = (int64_t)tot[p][0];
-    rec[(size_t)ri * nd + p] = r;
+    rec[((size_t)ri * gridDim.x + blockIdx.x) * nd_max + p] = r;
   }
   for (int p = 0; p < nd; ++p) q[p] = qn[p];
-  st[0] = i + 1;
+  if (advance) st[0] = i + 1;
+}
+
+__global__ void k_step_advance(uint32_t* __restrict__ st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st[0] = st[0] + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1289,6 +1337,12 @@ static int fill_params(const od_config* c, int n_pad, DevParams* P) {
   P->distribution = c->distribution;
   P->use_bottom = c->use_bottom;
   P->dust_mask = (c->n_dust > 0) ? ((1u << c->n_dust) - 1u) : 0u;
+  P->ion_off = 0;
+  P->stride = n_pad;
+  P->blk0 = 0;
+  P->blk1 = n_pad / 256;
+  P->q_off = 0;
+  P->sim_id = (uint32_t)c->sim_id;
   P->seed = c->seed;
   P->R = c->R;
   P->H = c->H;
@@ -1583,12 +1637,14 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
     if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(256), 0, c->stream, (const float4*)pj, c->n_blocks,
-                       c->sym_ch, (const int2*)c->wg_tab, a, linv, c->sym_tp, c->sym_sp);
+                       c->sym_ch, (const int2*)c->wg_tab, a, linv, (const int*)nullptr, (const float*)nullptr,
+                       c->sym_tp, c->sym_sp);
     if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
     hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
-                       (const float4*)pj, c->n_pad, a, linv, c->sym_dg);
+                       (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
     hipLaunchKernelGGL(k_sym_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->sym_dg, c->sym_tp,
-                       c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, F);
+                       c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, (const int*)nullptr,
+                       (const double*)nullptr, F);
     return OD_OK;
   }
   double* out = (c->n_chunks > 1) ? c->Fpart : F;
@@ -1666,8 +1722,8 @@ template <class T>
 static int launch_step_t(od_ctx* c, bool timed) {
   using PV = typename Pos<T>::V;
   const int nb = c->nb;
-  hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
-                     (PV*)c->pi, (PV*)c->pj, c->flags, c->block_cnt, c->err);
+  hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->X, c->V,
+                     c->F, c->q, c->st, (PV*)c->pi, (PV*)c->pj, c->flags, c->block_cnt, c->err);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->block_cnt, nb, c->block_off, c->n_flag);
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, c->stream, c->flags, c->block_cnt, c->block_off, c->n,
                      (const PV*)c->pi, c->slot_of, c->list_ion, (PV*)c->list_old);
@@ -1681,10 +1737,12 @@ static int launch_step_t(od_ctx* c, bool timed) {
                          (const PV*)c->spos, c->cell_start, c->n_flag, (const PV*)c->list_old,
                          (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
     else
-      hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pi, c->n_flag,
-                         (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
+      hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const int*)nullptr,
+                         (const PV*)c->pi, c->n_flag, (const PV*)c->list_old, c->flags, c->list_ion, dust_bits,
+                         c->rowD);
   }
-  hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->X, c->V, c->F, c->q, c->st,
+  hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
+                     c->X, c->V, c->F, c->q, c->st,
                      (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
                      (PV*)c->list_new, c->face, c->rowD, c->partials);
   if (c->cut)
@@ -1692,11 +1750,12 @@ static int launch_step_t(od_ctx* c, bool timed) {
                        c->cell_start, c->n_flag, (const PV*)c->list_new, (const PV*)c->list_new, c->flags,
                        c->list_ion, 0u, c->rowF);
   else
-    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->pj, c->n_flag,
-                       (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
-  hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, c->F, (const PV*)c->pj, c->flags,
-                     c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
-  hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, nb, c->q, c->st, c->rec);
+    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const int*)nullptr,
+                       (const PV*)c->pj, c->n_flag, (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
+  hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
+                     c->F, (const PV*)c->pj, c->flags, c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
+  hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, c->q, c->st, c->rec,
+                     std::max(1, c->cfg.n_dust), 1);
   return OD_OK;
 }
 
@@ -2066,3 +2125,506 @@ extern "C" int od_abi_sizes(int64_t* config_size, int64_t* record_size) {
 extern "C" const char* od_last_error(void) { return g_err.c_str(); }
 
 extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950, ABI " "1"; }
+
+// ---------------------------------------------------------------------------
+// Batched parameter sweeps: many independent simulations with uneven ion and
+// dust counts advanced together.  Their ions are concatenated (each padded to a
+// multiple of 1024), a block -> simulation table selects each 256-ion block's
+// parameter record, and the all-pairs work of all simulations is one launch
+// over a list of items (simulation, 512 targets, source chunk) so small and
+// large simulations fill the GPU together.  The step is the same sequence of
+// kernels as a single simulation's, each launched once for all simulations,
+// and can be replayed as a captured hipGraph.
+// ---------------------------------------------------------------------------
+constexpr int BATCH_SRC_CHUNK = 8192;  // sources per pair item
+
+struct PairItem {
+  int sim;       // simulation
+  int tgt;       // first of the 512 targets (global ion index)
+  int src0, src1;  // source range (global ion indices)
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pair_items(const DevParams* __restrict__ Pall,
+                                                    const PairItem* __restrict__ items,
+                                                    const typename Pos<T>::V* __restrict__ pj,
+                                                    double* __restrict__ part) {
+  const PairItem it = items[blockIdx.x];
+  const DevParams* P = Pall + it.sim;
+  const typename Pos<T>::V a = pj[it.tgt + threadIdx.x];
+  const typename Pos<T>::V b = pj[it.tgt + 256 + threadIdx.x];
+  Acc<T> ax0, ay0, az0, ax1, ay1, az1;
+  for (int jt = it.src0; jt < it.src1; jt += TJ) {
+#pragma unroll 8
+    for (int j = jt; j < jt + TJ; ++j) {
+      const typename Pos<T>::V p = pj[j];
+      pair_acc<T>(*P, a, p, ax0, ay0, az0);
+      pair_acc<T>(*P, b, p, ax1, ay1, az1);
+    }
+    ax0.flush();
+    ay0.flush();
+    az0.flush();
+    ax1.flush();
+    ay1.flush();
+    az1.flush();
+  }
+  double* o = part + (size_t)blockIdx.x * 3 * 512;
+  o[threadIdx.x] = ax0.total();
+  o[512 + threadIdx.x] = ay0.total();
+  o[1024 + threadIdx.x] = az0.total();
+  o[256 + threadIdx.x] = ax1.total();
+  o[512 + 256 + threadIdx.x] = ay1.total();
+  o[1024 + 256 + threadIdx.x] = az1.total();
+}
+
+// F = pair_scale(sim) * sum of the ion's items (fixed order); group_items[g] =
+// {first item, count} of the 512-ion group g
+__global__ __launch_bounds__(256) void k_items_reduce(const DevParams* __restrict__ Pall,
+                                                      const int* __restrict__ blk_sim,
+                                                      const int2* __restrict__ group_items,
+                                                      const double* __restrict__ part, int n_tot,
+                                                      double* __restrict__ F) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_tot) return;
+  const DevParams* P = Pall + blk_sim[blockIdx.x];
+  const int2 gi = group_items[i >> 9];
+  const int off = i & 511;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int k = 0; k < gi.y; ++k) s += part[((size_t)(gi.x + k) * 3 + c) * 512 + off];
+    F[(size_t)c * n_tot + i] = s * P->pair_scale;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_batch_load_pos(const DevParams* __restrict__ Pall,
+                                                        const int* __restrict__ blk_sim, const double* __restrict__ X,
+                                                        typename Pos<T>::V* __restrict__ pi,
+                                                        typename Pos<T>::V* __restrict__ pj) {
+  const DevParams* P = Pall + blk_sim[blockIdx.x];
+  const int np = P->stride;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const typename Pos<T>::V p = (i - P->ion_off < P->n) ? Pos<T>::make(X[i], X[np + i], X[2 * np + i], P->pos_scale)
+                                                      : Pos<T>::park();
+  pi[i] = p;
+  pj[i] = p;
+}
+
+struct od_batch {
+  int n_sims = 0, dev = 0, prec = OD_PREC_FP32C;
+  hipStream_t stream = nullptr;
+  std::vector<od_config> cfgs;
+  std::vector<int> off, npad;
+  int n_tot = 0, nb = 0, n_items = 0, rows_grid = 1024, rec_cap = 128;
+  size_t pos_bytes = 16;
+  DevParams* dP = nullptr;
+  int* blk_sim = nullptr;
+  PairItem* items = nullptr;
+  int2* group_items = nullptr;
+  double *X = nullptr, *V = nullptr, *F = nullptr, *q = nullptr, *partials = nullptr, *rowF = nullptr,
+         *rowD = nullptr, *part = nullptr;
+  void *pi = nullptr, *pj = nullptr, *list_old = nullptr, *list_new = nullptr;
+  uint32_t *flags = nullptr, *st = nullptr;
+  uint8_t* face = nullptr;
+  int *block_cnt = nullptr, *block_off = nullptr, *n_flag = nullptr, *slot_of = nullptr, *list_ion = nullptr,
+      *err = nullptr;
+  od_dust_record* rec = nullptr;
+  int64_t host_step = 0;
+  bool stale = true;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_steps = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double pair_ms = 0.0;
+  int64_t pair_launches = 0, pairs_per_launch = 0;
+  // symmetric FP32 pair kernel over all simulations (1024-ion blocks)
+  int sym = 0, sym_ch = 1, n_wg = 0, n_blocks = 0;
+  int2* wg_tab = nullptr;
+  int *wg_base = nullptr, *blk_begin = nullptr, *blk_end = nullptr;
+  float* blk_a = nullptr;
+  double *blk_scale = nullptr, *sym_tp = nullptr, *sym_dg = nullptr;
+  float* sym_sp = nullptr;
+};
+
+static void free_batch(od_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->dev);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  void* bufs[] = {b->dP, b->blk_sim, b->items, b->group_items, b->X, b->V, b->F, b->q, b->partials, b->rowF,
+                  b->rowD, b->part, b->pi, b->pj, b->list_old, b->list_new, b->flags, b->st, b->face, b->block_cnt,
+                  b->block_off, b->n_flag, b->slot_of, b->list_ion, b->err, b->rec, b->wg_tab, b->wg_base,
+                  b->blk_begin, b->blk_end, b->blk_a, b->blk_scale, b->sym_tp, b->sym_dg, b->sym_sp};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch** out) {
+  if (!out || !cfgs) return set_err(OD_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (n_sims < 1 || n_sims > 4096) return set_err(OD_ERR_ARG, "n_sims %d outside [1, 4096]", n_sims);
+  for (int s = 0; s < n_sims; ++s) {
+    CHKRET(check_config(&cfgs[s]));
+    if (cfgs[s].precision != cfgs[0].precision || cfgs[s].device != cfgs[0].device)
+      return set_err(OD_ERR_ARG, "all simulations of a batch share precision and device");
+    if (cfgs[s].cutoff > 0.0) return set_err(OD_ERR_ARG, "batched sweeps run all-pairs forces (no cutoff)");
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return set_err(OD_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (cfgs[0].device < 0 || cfgs[0].device >= ndev) return set_err(OD_ERR_ARG, "device %d out of range", cfgs[0].device);
+  od_batch* b = new od_batch();
+  auto fail = [&](int r) {
+    free_batch(b);
+    return r;
+  };
+  b->n_sims = n_sims;
+  b->dev = cfgs[0].device;
+  b->prec = cfgs[0].precision;
+  b->pos_bytes = (b->prec == OD_PREC_FP32C) ? sizeof(float4) : sizeof(double4);
+  b->cfgs.assign(cfgs, cfgs + n_sims);
+  int64_t tot = 0;
+  for (int s = 0; s < n_sims; ++s) {
+    const int np = (cfgs[s].n_ions + 1023) / 1024 * 1024;
+    b->off.push_back((int)tot);
+    b->npad.push_back(np);
+    tot += np;
+  }
+  if (tot > (1 << 28)) return fail(set_err(OD_ERR_ARG, "batch too large (%lld ions)", (long long)tot));
+  b->n_tot = (int)tot;
+  b->nb = b->n_tot / 256;
+  // parameter records and tables
+  std::vector<DevParams> hp(n_sims);
+  std::vector<int> blk_sim(b->nb);
+  std::vector<PairItem> items;
+  std::vector<int2> groups(b->n_tot / 512);
+  for (int s = 0; s < n_sims; ++s) {
+    fill_params(&cfgs[s], b->npad[s], &hp[s]);
+    hp[s].ion_off = b->off[s];
+    hp[s].stride = b->n_tot;
+    hp[s].blk0 = b->off[s] / 256;
+    hp[s].blk1 = (b->off[s] + b->npad[s]) / 256;
+    hp[s].q_off = s * OD_MAX_DUST;
+    for (int k = hp[s].blk0; k < hp[s].blk1; ++k) blk_sim[k] = s;
+    for (int t = 0; t < b->npad[s]; t += 512) {
+      groups[(b->off[s] + t) / 512] = make_int2((int)items.size(), 0);
+      for (int j = 0; j < b->npad[s]; j += BATCH_SRC_CHUNK) {
+        PairItem it;
+        it.sim = s;
+        it.tgt = b->off[s] + t;
+        it.src0 = b->off[s] + j;
+        it.src1 = b->off[s] + std::min(b->npad[s], j + BATCH_SRC_CHUNK);
+        items.push_back(it);
+        groups[(b->off[s] + t) / 512].y += 1;
+      }
+    }
+  }
+  b->n_items = (int)items.size();
+  b->sym = (b->prec == OD_PREC_FP32C && cfgs[0].pair_mode != OD_PAIR_ROWS) ? 1 : 0;
+  const size_t np = b->n_tot;
+  int rc = OD_OK;
+  if (hipSetDevice(b->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "hipEventCreate failed"));
+  for (int s = 0; s < n_sims; ++s) b->pairs_per_launch += (int64_t)cfgs[s].n_ions * (cfgs[s].n_ions - 1);
+  if ((rc = dalloc(&b->dP, n_sims * sizeof(DevParams))) || (rc = dalloc(&b->blk_sim, b->nb * 4)) ||
+      (rc = dalloc(&b->items, items.size() * sizeof(PairItem))) ||
+      (rc = dalloc(&b->group_items, groups.size() * sizeof(int2))) || (rc = dalloc(&b->X, 3 * np * 8)) ||
+      (rc = dalloc(&b->V, 3 * np * 8)) || (rc = dalloc(&b->F, 3 * np * 8)) ||
+      (rc = dalloc(&b->q, (size_t)n_sims * OD_MAX_DUST * 8)) ||
+      (rc = dalloc(&b->partials, (size_t)b->nb * OD_MAX_DUST * 8 * 8)) || (rc = dalloc(&b->rowF, 3 * np * 8)) ||
+      (rc = dalloc(&b->rowD, 3 * np * 8)) || (rc = dalloc(&b->part, b->sym ? 8 : (size_t)b->n_items * 3 * 512 * 8)) ||
+      (rc = dalloc(&b->pi, np * b->pos_bytes)) || (rc = dalloc(&b->pj, np * b->pos_bytes)) ||
+      (rc = dalloc(&b->list_old, np * b->pos_bytes)) || (rc = dalloc(&b->list_new, np * b->pos_bytes)) ||
+      (rc = dalloc(&b->flags, np * 4)) || (rc = dalloc(&b->st, 16)) || (rc = dalloc(&b->face, np)) ||
+      (rc = dalloc(&b->block_cnt, b->nb * 4)) || (rc = dalloc(&b->block_off, b->nb * 4)) ||
+      (rc = dalloc(&b->n_flag, 4)) || (rc = dalloc(&b->slot_of, np * 4)) || (rc = dalloc(&b->list_ion, np * 4)) ||
+      (rc = dalloc(&b->err, 4)) ||
+      (rc = dalloc(&b->rec, (size_t)b->rec_cap * n_sims * OD_MAX_DUST * sizeof(od_dust_record))))
+    return fail(rc);
+  std::vector<double> q0((size_t)n_sims * OD_MAX_DUST, 0.0);
+  for (int s = 0; s < n_sims; ++s)
+    for (int p = 0; p < cfgs[s].n_dust; ++p) q0[(size_t)s * OD_MAX_DUST + p] = cfgs[s].dust_q0[p];
+  uint32_t st0[4] = {0, 0, 0, 0};
+  if (hipMemcpy(b->dP, hp.data(), n_sims * sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->blk_sim, blk_sim.data(), b->nb * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->items, items.data(), items.size() * sizeof(PairItem), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->group_items, groups.data(), groups.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->q, q0.data(), q0.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->st, st0, sizeof(st0), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->X, 0, 3 * np * 8) != hipSuccess || hipMemset(b->V, 0, 3 * np * 8) != hipSuccess ||
+      hipMemset(b->F, 0, 3 * np * 8) != hipSuccess || hipMemset(b->err, 0, 4) != hipSuccess ||
+      hipMemset(b->n_flag, 0, 4) != hipSuccess || hipMemset(b->flags, 0, np * 4) != hipSuccess ||
+      hipMemset(b->face, 0, np) != hipSuccess ||
+      hipMemset(b->rec, 0, (size_t)b->rec_cap * n_sims * OD_MAX_DUST * sizeof(od_dust_record)) != hipSuccess)
+    return fail(set_err(OD_ERR_HIP, "batch initialisation failed"));
+  if (b->sym) {
+    // triangular schedule over every simulation's 1024-ion blocks
+    const int NB = b->n_tot / SYM_TB;
+    std::vector<int> bb(NB), be(NB);
+    std::vector<float> ba(NB);
+    std::vector<double> bs(NB);
+    for (int s = 0; s < n_sims; ++s)
+      for (int X = b->off[s] / SYM_TB; X < (b->off[s] + b->npad[s]) / SYM_TB; ++X) {
+        bb[X] = b->off[s] / SYM_TB;
+        be[X] = (b->off[s] + b->npad[s]) / SYM_TB;
+        ba[X] = hp[s].a32;
+        bs[X] = hp[s].pair_scale;
+      }
+    auto count = [&](int ch) {
+      int64_t n = 0;
+      for (int X = 0; X < NB; ++X) n += (be[X] - 1 - X + ch - 1) / ch;
+      return n;
+    };
+    int ch = 16;
+    while (ch > 1 && count(ch) < 4096) ch /= 2;
+    std::vector<int2> tab;
+    std::vector<int> base(NB + 1);
+    for (int X = 0; X < NB; ++X) {
+      base[X] = (int)tab.size();
+      const int nk = (be[X] - 1 - X + ch - 1) / ch;
+      for (int k = 0; k < nk; ++k) tab.push_back(make_int2(X, k));
+    }
+    base[NB] = (int)tab.size();
+    // ordered pairs of real ions the symmetric launch covers (different blocks)
+    b->pairs_per_launch = 0;
+    for (int s = 0; s < n_sims; ++s) {
+      const int64_t n = cfgs[s].n_ions;
+      int64_t diag = 0;
+      for (int64_t x0 = 0; x0 < n; x0 += SYM_TB) {
+        const int64_t m = std::min<int64_t>(SYM_TB, n - x0);
+        diag += m * (m - 1);
+      }
+      b->pairs_per_launch += n * (n - 1) - diag;
+    }
+    b->n_blocks = NB;
+    b->sym_ch = ch;
+    b->n_wg = (int)tab.size();
+    const size_t n_wg = std::max<size_t>(tab.size(), 1);
+    if ((rc = dalloc(&b->wg_tab, n_wg * sizeof(int2))) || (rc = dalloc(&b->wg_base, base.size() * 4)) ||
+        (rc = dalloc(&b->blk_begin, NB * 4)) || (rc = dalloc(&b->blk_end, NB * 4)) ||
+        (rc = dalloc(&b->blk_a, NB * 4)) || (rc = dalloc(&b->blk_scale, NB * 8)) ||
+        (rc = dalloc(&b->sym_tp, n_wg * 3 * SYM_TB * 8)) || (rc = dalloc(&b->sym_sp, n_wg * 3 * ch * SYM_TB * 4)) ||
+        (rc = dalloc(&b->sym_dg, 3 * np * 8)))
+      return fail(rc);
+    if ((!tab.empty() &&
+         hipMemcpy(b->wg_tab, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(b->wg_base, base.data(), base.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->blk_begin, bb.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->blk_end, be.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->blk_a, ba.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->blk_scale, bs.data(), NB * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(OD_ERR_HIP, "batch initialisation failed"));
+  }
+  *out = b;
+  return OD_OK;
+}
+
+extern "C" int od_batch_destroy(od_batch* b) {
+  free_batch(b);
+  return OD_OK;
+}
+
+extern "C" int od_batch_set_ions(od_batch* b, int32_t sim, const double* x, const double* v, int64_t n) {
+  if (!b || !x || !v) return set_err(OD_ERR_ARG, "null argument");
+  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
+  if (n != b->cfgs[sim].n_ions)
+    return set_err(OD_ERR_ARG, "simulation %d holds %d ions, got %lld", sim, b->cfgs[sim].n_ions, (long long)n);
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(v[i])) return set_err(OD_ERR_NONFINITE, "non-finite ion state");
+  HIPCHK(hipSetDevice(b->dev));
+  const size_t np = b->n_tot;
+  for (int k = 0; k < 3; ++k) {
+    std::vector<double> cx(n), cv(n);
+    for (int64_t i = 0; i < n; ++i) {
+      cx[i] = x[3 * i + k];
+      cv[i] = v[3 * i + k];
+    }
+    HIPCHK(hipMemcpy(b->X + k * np + b->off[sim], cx.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->V + k * np + b->off[sim], cv.data(), n * 8, hipMemcpyHostToDevice));
+  }
+  b->stale = true;
+  return OD_OK;
+}
+
+extern "C" int od_batch_get_ions(od_batch* b, int32_t sim, double* x, double* v) {
+  if (!b) return set_err(OD_ERR_ARG, "null batch");
+  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
+  HIPCHK(hipSetDevice(b->dev));
+  const int n = b->cfgs[sim].n_ions;
+  const size_t np = b->n_tot;
+  std::vector<double> c(n);
+  for (int k = 0; k < 3; ++k) {
+    if (x) {
+      HIPCHK(hipMemcpy(c.data(), b->X + k * np + b->off[sim], n * 8, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) x[3 * (size_t)i + k] = c[i];
+    }
+    if (v) {
+      HIPCHK(hipMemcpy(c.data(), b->V + k * np + b->off[sim], n * 8, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) v[3 * (size_t)i + k] = c[i];
+    }
+  }
+  return OD_OK;
+}
+
+template <class T>
+static void batch_pair(od_batch* b, bool timed) {
+  using PV = typename Pos<T>::V;
+  if (b->sym) {
+    const float linv = (float)(1.0 / 1.4426950408889634);
+    if (timed) hipEventRecord(b->ev0, b->stream);
+    if (b->n_wg > 0)
+      hipLaunchKernelGGL(k_pair32_sym, dim3(b->n_wg), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_blocks,
+                         b->sym_ch, (const int2*)b->wg_tab, 0.0f, linv, (const int*)b->blk_end,
+                         (const float*)b->blk_a, b->sym_tp, b->sym_sp);
+    if (timed) hipEventRecord(b->ev1, b->stream);
+    hipLaunchKernelGGL(k_pair32_diag, dim3(b->n_tot / 256), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_tot,
+                       0.0f, linv, (const float*)b->blk_a, b->sym_dg);
+    hipLaunchKernelGGL(k_sym_reduce, dim3(b->n_tot / 256), dim3(256), 0, b->stream, b->sym_dg, b->sym_tp,
+                       b->sym_sp, b->wg_base, b->n_blocks, b->sym_ch, b->n_tot, 0.0, (const int*)b->blk_begin,
+                       (const double*)b->blk_scale, b->F);
+    return;
+  }
+  if (timed) hipEventRecord(b->ev0, b->stream);
+  hipLaunchKernelGGL(k_pair_items<T>, dim3(b->n_items), dim3(256), 0, b->stream, b->dP, b->items, (const PV*)b->pj,
+                     b->part);
+  if (timed) hipEventRecord(b->ev1, b->stream);
+  hipLaunchKernelGGL(k_items_reduce, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->group_items,
+                     b->part, b->n_tot, b->F);
+}
+
+template <class T>
+static int batch_step_t(od_batch* b, bool timed) {
+  using PV = typename Pos<T>::V;
+  const int nb = b->nb;
+  hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X, b->V, b->F, b->q,
+                     b->st, (PV*)b->pi, (PV*)b->pj, b->flags, b->block_cnt, b->err);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, b->stream, b->block_cnt, nb, b->block_off, b->n_flag);
+  hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, b->stream, b->flags, b->block_cnt, b->block_off, b->n_tot,
+                     (const PV*)b->pi, b->slot_of, b->list_ion, (PV*)b->list_old);
+  batch_pair<T>(b, timed);
+  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pi,
+                     b->n_flag, (const PV*)b->list_old, b->flags, b->list_ion, (uint32_t)((1u << OD_MAX_DUST) - 1u),
+                     b->rowD);
+  hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->X, b->V,
+                     b->F, b->q, b->st, (const PV*)b->pi, (PV*)b->pj, b->flags, b->n_flag, b->slot_of,
+                     (const PV*)b->list_old, (PV*)b->list_new, b->face, b->rowD, b->partials);
+  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pj,
+                     b->n_flag, (const PV*)b->list_new, b->flags, b->list_ion, 0u, b->rowF);
+  hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->F,
+                     (const PV*)b->pj, b->flags, b->n_flag, (const PV*)b->list_new, b->slot_of, b->rowF);
+  hipLaunchKernelGGL(k_dust, dim3(b->n_sims), dim3(256), 0, b->stream, b->dP, b->partials, b->q, b->st, b->rec,
+                     (int)OD_MAX_DUST, 0);
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->st);
+  return OD_OK;
+}
+
+static int batch_step(od_batch* b, bool timed) {
+  return (b->prec == OD_PREC_FP32C) ? batch_step_t<float>(b, timed) : batch_step_t<double>(b, timed);
+}
+
+template <class T>
+static int batch_forces_t(od_batch* b) {
+  using PV = typename Pos<T>::V;
+  hipLaunchKernelGGL(k_batch_load_pos<T>, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X,
+                     (PV*)b->pi, (PV*)b->pj);
+  batch_pair<T>(b, false);
+  return OD_OK;
+}
+
+extern "C" int od_batch_run(od_batch* b, int64_t nsteps, od_dust_record* out, int32_t flags) {
+  if (!b) return set_err(OD_ERR_ARG, "null batch");
+  if (nsteps < 0) return set_err(OD_ERR_ARG, "negative step count");
+  if (b->host_step + nsteps > 0xFFFFFFFFll) return set_err(OD_ERR_ARG, "step counter would exceed u32");
+  HIPCHK(hipSetDevice(b->dev));
+  if (b->stale) {
+    CHKRET((b->prec == OD_PREC_FP32C) ? batch_forces_t<float>(b) : batch_forces_t<double>(b));
+    HIPCHK(hipGetLastError());
+    b->stale = false;
+  }
+  const bool profile = (flags & OD_RUN_PROFILE_PAIR) != 0;
+  const bool graph = (flags & OD_RUN_GRAPH) != 0 && !profile;
+  int64_t done = 0;
+  const size_t rec_step = (size_t)b->n_sims * OD_MAX_DUST;
+  while (done < nsteps) {
+    const int chunk = (int)std::min<int64_t>(b->rec_cap, nsteps - done);
+    uint32_t base = (uint32_t)b->host_step;
+    HIPCHK(hipMemcpyAsync(b->st + 1, &base, 4, hipMemcpyHostToDevice, b->stream));
+    int left = chunk;
+    if (graph && chunk >= 4) {
+      const int gs = std::min(chunk, 16);
+      if (!(b->gexec && b->graph_steps == gs)) {
+        if (b->gexec) hipGraphExecDestroy(b->gexec);
+        if (b->graph) hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = OD_OK;
+        for (int s = 0; s < gs && rc == OD_OK; ++s) rc = batch_step(b, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != OD_OK) return rc;
+        if (e != hipSuccess) return set_err(OD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+        b->graph_steps = gs;
+      }
+      while (left >= gs) {
+        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        left -= gs;
+      }
+    }
+    for (; left > 0; --left) {
+      CHKRET(batch_step(b, profile));
+      HIPCHK(hipGetLastError());
+      if (profile) {
+        HIPCHK(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->pair_ms += ms;
+        b->pair_launches += 1;
+      }
+    }
+    if (out)
+      HIPCHK(hipMemcpyAsync(out + done * rec_step, b->rec, (size_t)chunk * rec_step * sizeof(od_dust_record),
+                            hipMemcpyDeviceToHost, b->stream));
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, b->err, 4, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->host_step += chunk;
+    done += chunk;
+    if (herr) return set_err(OD_ERR_NONFINITE, "non-finite ion state in the batch before step %lld",
+                             (long long)b->host_step);
+  }
+  return OD_OK;
+}
+
+extern "C" int od_batch_get_step_info(od_batch* b, int32_t sim, uint32_t* flags, uint8_t* face) {
+  if (!b) return set_err(OD_ERR_ARG, "null batch");
+  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
+  HIPCHK(hipSetDevice(b->dev));
+  const int n = b->cfgs[sim].n_ions;
+  if (flags) HIPCHK(hipMemcpy(flags, b->flags + b->off[sim], (size_t)n * 4, hipMemcpyDeviceToHost));
+  if (face) HIPCHK(hipMemcpy(face, b->face + b->off[sim], (size_t)n, hipMemcpyDeviceToHost));
+  return OD_OK;
+}
+
+extern "C" int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* launches, int64_t* pairs_per_launch,
+                                    int32_t reset) {
+  if (!b) return set_err(OD_ERR_ARG, "null batch");
+  if (total_ms) *total_ms = b->pair_ms;
+  if (launches) *launches = b->pair_launches;
+  if (pairs_per_launch) *pairs_per_launch = b->pairs_per_launch;
+  if (reset) {
+    b->pair_ms = 0.0;
+    b->pair_launches = 0;
+  }
+  return OD_OK;
+}

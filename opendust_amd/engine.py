@@ -69,7 +69,7 @@ class Config(ctypes.Structure):
         ("j_split", ctypes.c_int32),
         ("pair_mode", ctypes.c_int32),
         ("sym_chunk", ctypes.c_int32),
-        ("reserved_", ctypes.c_int32),
+        ("sim_id", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("R", ctypes.c_double), ("H", ctypes.c_double), ("dt", ctypes.c_double),
         ("r_D_e", ctypes.c_double), ("r_0", ctypes.c_double),
@@ -141,6 +141,8 @@ EXPORTS = [
     "od_get_cells", "od_cutoff_pair_count",
     "od_trap_potential_grid", "od_trap_samples_host", "od_abi_sizes", "od_destroy",
     "od_last_error", "od_build_info",
+    "od_batch_create", "od_batch_set_ions", "od_batch_get_ions", "od_batch_run", "od_batch_get_step_info",
+    "od_batch_pair_timing", "od_batch_destroy",
 ]
 
 
@@ -176,6 +178,13 @@ def load_library(path=LIB_PATH):
                                          ctypes.c_int64, dp]
     lib.od_abi_sizes.argtypes = [P(ctypes.c_int64), P(ctypes.c_int64)]
     lib.od_destroy.argtypes = [c_ctx]
+    lib.od_batch_create.argtypes = [P(Config), ctypes.c_int32, P(c_ctx)]
+    lib.od_batch_set_ions.argtypes = [c_ctx, ctypes.c_int32, dp, dp, ctypes.c_int64]
+    lib.od_batch_get_ions.argtypes = [c_ctx, ctypes.c_int32, dp, dp]
+    lib.od_batch_run.argtypes = [c_ctx, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32]
+    lib.od_batch_get_step_info.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_uint32), P(ctypes.c_uint8)]
+    lib.od_batch_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
+    lib.od_batch_destroy.argtypes = [c_ctx]
     lib.od_last_error.restype = ctypes.c_char_p
     lib.od_build_info.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -321,3 +330,69 @@ class Engine:
         f = np.empty_like(x)
         _check(self.lib.od_pair_forces(self._ctx, _dptr(x), _dptr(f)), "od_pair_forces")
         return f
+
+
+class Batch:
+    """Independent simulations advanced together on one device (od_batch_*).
+    cfgs: list of Config; cfgs[s].sim_id should differ between simulations."""
+
+    def __init__(self, cfgs):
+        self.lib = load_library()
+        arr = (Config * len(cfgs))()
+        for s, c in enumerate(cfgs):
+            c.struct_size = ctypes.sizeof(Config)
+            arr[s] = c
+        self.cfgs = list(cfgs)
+        self.n_sims = len(cfgs)
+        self._b = ctypes.c_void_p()
+        _check(self.lib.od_batch_create(arr, self.n_sims, ctypes.byref(self._b)), "od_batch_create")
+
+    def close(self):
+        if self._b:
+            self.lib.od_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ions(self, sim, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        _check(self.lib.od_batch_set_ions(self._b, int(sim), _dptr(x), _dptr(v), x.shape[0]), "od_batch_set_ions")
+
+    def get_ions(self, sim):
+        n = self.cfgs[sim].n_ions
+        x = np.empty((n, 3))
+        v = np.empty((n, 3))
+        _check(self.lib.od_batch_get_ions(self._b, int(sim), _dptr(x), _dptr(v)), "od_batch_get_ions")
+        return x, v
+
+    def run(self, nsteps, flags=0, records=True):
+        """nsteps steps of all simulations; records [nsteps][n_sims][OD_MAX_DUST]."""
+        if not records:
+            _check(self.lib.od_batch_run(self._b, int(nsteps), None, int(flags)), "od_batch_run")
+            return None
+        rec = np.zeros((nsteps, self.n_sims, MAX_DUST), dtype=RECORD_DTYPE)
+        _check(self.lib.od_batch_run(self._b, int(nsteps), rec.ctypes.data_as(ctypes.c_void_p), int(flags)),
+               "od_batch_run")
+        return rec
+
+    def step_info(self, sim):
+        n = self.cfgs[sim].n_ions
+        flags = np.empty(n, dtype=np.uint32)
+        face = np.empty(n, dtype=np.uint8)
+        _check(self.lib.od_batch_get_step_info(self._b, int(sim), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               face.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+               "od_batch_get_step_info")
+        return flags, face
+
+    def pair_timing(self, reset=False):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        pairs = ctypes.c_int64()
+        _check(self.lib.od_batch_pair_timing(self._b, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs),
+                                             int(bool(reset))), "od_batch_pair_timing")
+        return ms.value, n.value, pairs.value

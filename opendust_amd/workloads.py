@@ -11,6 +11,10 @@ and initial state without running ``simulate()``.
                   at z = -1, 0, +1 r_D_e, R = 6, H = 14 r_D_e, N = 2^20.
   cutoff_large    single dust, N = 2^22, R = 10, H = 20 r_D_e, ion-ion force
                   truncated at 2 r_D_e through the cell list.
+  sweep           64 independent single-flow simulations: N log-uniform in
+                  [2^12, 2^17], M uniform in [0.5, 2], 1-4 dusts on the axis
+                  with spacing uniform in [0.5, 2] r_D_e (centred), q0 uniform
+                  in [-3e4, -1e4] e, the rest as single_dust; 5000 steps.
   piel_fig1       the tutorial / Piel 2017 Fig. 1 setting (n = 1e14 m^-3,
                   R = 3, H = 6 r_D_e, r = 58.8 um at z = -r_D_e, N = 2^16).
 """
@@ -60,6 +64,28 @@ def dust_chain(N=2 ** 20, n=20000, precision="fp32c", seed=1, **kw):
     dust = [DustParticle(0.0, 0.0, z * lam, 0.05 * lam, "consistent", -2e4 * abs(pp.e)) for z in (-1.0, 0.0, 1.0)]
     return OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dust, "fieldDriven", precision=precision, seed=seed,
                     **kw)
+
+
+def sweep(n_sims=64, n=5000, precision="fp32c", seed=7, log2_n=(12.0, 17.0), **kw):
+    """OpenDust objects of the parameter sweep (see the module docstring); the
+    parameters are drawn from NumPy RandomState(seed), simulation s gets
+    Philox key seed + s."""
+    rs = np.random.RandomState(seed)
+    ods = []
+    for s in range(n_sims):
+        N = int(round(2.0 ** rs.uniform(*log2_n)))
+        mach = rs.uniform(0.5, 2.0)
+        nd = int(rs.randint(1, 5))
+        spacing = rs.uniform(0.5, 2.0)
+        q0 = rs.uniform(-3e4, -1e4)
+        pp = PlasmaParametersInSIUnitsMaxwell(T_I, T_E, N_INF, mach, M_AR)
+        lam = pp.r_D_e
+        sp = SimulationParametersInSIUnits(5 * lam, 10 * lam, N, int(n), 0.01 / pp.w_p_i, pp)
+        zs = [(k - (nd - 1) / 2.0) * spacing * lam for k in range(nd)]
+        dust = [DustParticle(0.0, 0.0, z, 0.05 * lam, "consistent", q0 * abs(pp.e)) for z in zs]
+        ods.append(OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dust, "Maxwellian", precision=precision,
+                            seed=seed + s, **kw))
+    return ods
 
 
 def piel_fig1(N=2 ** 16, n=3000, charge="consistent", q_e=0.0, precision="fp32c", seed=1, **kw):

@@ -66,9 +66,10 @@ void orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4]) {
 enum { ST_FACE = 0, ST_TOPV = 1, ST_TOPX = 2, ST_BOTV = 3, ST_BOTX = 4, ST_SIDEX = 5, ST_SIDEV = 6, ST_COLL = 7,
        ST_TRAP = 16 };
 
-static void draw(uint32_t ion, uint32_t step, uint32_t stream, uint32_t sub, uint64_t seed, uint32_t out[4]) {
-  const uint32_t ctr[4] = {ion, step, 0u, stream | (sub << 8)};
-  orc_philox(ctr, seed, out);
+/* counter {ion, step, simulation, stream | sub << 8}, key = seed */
+static void draw(const od_config* c, uint32_t ion, uint32_t step, uint32_t stream, uint32_t sub, uint32_t out[4]) {
+  const uint32_t ctr[4] = {ion, step, (uint32_t)c->sim_id, stream | (sub << 8)};
+  orc_philox(ctr, c->seed, out);
 }
 
 static double unif53(uint32_t a, uint32_t b) {
@@ -186,7 +187,7 @@ static void total_forces(const od_config* c, int n, const double* x, const doubl
 int orc_reinject(const od_config* c, uint32_t ion, uint32_t step, double* x, double* v) {
   uint32_t w[4];
   const double pi2 = 2.0 * 3.141592653589793;
-  draw(ion, step, ST_FACE, 0, c->seed, w);
+  draw(c, ion, step, ST_FACE, 0, w);
   const uint64_t u = w[0];
   const uint64_t thr_side = c->use_bottom ? c->thr_side : 4294967296ull;
   int face = (u < c->thr_top) ? 1 : (u < thr_side ? 2 : 3);
@@ -194,28 +195,28 @@ int orc_reinject(const od_config* c, uint32_t ion, uint32_t step, double* x, dou
     const int top = face == 1;
     const od_table* tab = top ? &c->top : &c->bottom;
     double g1, g2;
-    draw(ion, step, top ? ST_TOPV : ST_BOTV, 0, c->seed, w);
+    draw(c, ion, step, top ? ST_TOPV : ST_BOTV, 0, w);
     gauss_pair(w, &g1, &g2);
-    draw(ion, step, top ? ST_TOPV : ST_BOTV, 1, c->seed, w);
+    draw(c, ion, step, top ? ST_TOPV : ST_BOTV, 1, w);
     const double uz = unif53(w[0], w[1]);
     v[0] = c->v_T * g1;
     v[1] = c->v_T * g2;
     v[2] = orc_table_eval(tab, uz);
-    draw(ion, step, top ? ST_TOPX : ST_BOTX, 0, c->seed, w);
+    draw(c, ion, step, top ? ST_TOPX : ST_BOTX, 0, w);
     const double ux = unif53(w[0], w[1]), uy = unif53(w[2], w[3]);
     x[0] = c->R * sqrt(ux) * cos(pi2 * uy);
     x[1] = c->R * sqrt(ux) * sin(pi2 * uy);
     x[2] = top ? -c->H / 2 : c->H / 2;
   } else {
-    draw(ion, step, ST_SIDEX, 0, c->seed, w);
+    draw(c, ion, step, ST_SIDEX, 0, w);
     const double uy = unif53(w[0], w[1]), uz = unif53(w[2], w[3]);
     x[0] = c->R * cos(pi2 * uy);
     x[1] = c->R * sin(pi2 * uy);
     x[2] = uz * c->H - c->H / 2;
     double gy, gz;
-    draw(ion, step, ST_SIDEV, 0, c->seed, w);
+    draw(c, ion, step, ST_SIDEV, 0, w);
     gauss_pair(w, &gy, &gz);
-    draw(ion, step, ST_SIDEV, 1, c->seed, w);
+    draw(c, ion, step, ST_SIDEV, 1, w);
     const double us = unif53(w[0], w[1]);
     const double vs = orc_table_eval(&c->side, us);
     v[0] = -vs * (x[0] / c->R) + c->v_T * gy * (x[1] / c->R);
@@ -228,14 +229,14 @@ int orc_reinject(const od_config* c, uint32_t ion, uint32_t step, double* x, dou
 /* collisions of one ion's velocity (fieldDriven) */
 void orc_collide(const od_config* c, uint32_t ion, uint32_t step, double* v) {
   uint32_t w[4];
-  draw(ion, step, ST_COLL, 0, c->seed, w);
+  draw(c, ion, step, ST_COLL, 0, w);
   int hit[3];
   for (int k = 0; k < 3; ++k) hit[k] = (uint64_t)w[k] < c->thr_coll;
   if (!(hit[0] || hit[1] || hit[2])) return;
   double g[4];
-  draw(ion, step, ST_COLL, 1, c->seed, w);
+  draw(c, ion, step, ST_COLL, 1, w);
   gauss_pair(w, &g[0], &g[1]);
-  draw(ion, step, ST_COLL, 2, c->seed, w);
+  draw(c, ion, step, ST_COLL, 2, w);
   gauss_pair(w, &g[2], &g[3]);
   for (int k = 0; k < 3; ++k)
     if (hit[k]) v[k] = c->v_T * g[k];

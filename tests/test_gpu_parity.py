@@ -11,8 +11,11 @@ Tolerances (stated here, checked below):
     top of a few FP32 ulps of the arithmetic; the compensated sums add nothing
     visible.  Close pairs dominate the force, hence: pair forces norm-wise rel
     <= 1e-5, per ion |dF_i| <= 1e-4 |F_i| + 1e-5 rms|F|; after one step
-    |dx| <= 1e-8 R, |dv| <= 1e-4 max|v|, dust records rel <= 1e-5 of their
-    scale.
+    per ion |dx_i| <= dt (dt/2) / m_s (1e-4 |F_i| + 1e-5 rms|F|) (the drift of
+    that force error; reinjected ions to FP64 rounding), |dv| <= 1e-4 max|v|,
+    dust charges, collection and dust-dust forces rel <= 1e-5 of their scale,
+    orbit sums and forces within the propagation of the per-ion position
+    bounds through the orbit term (conftest.check_records).
   * Integer/index outputs (flags, reinjection faces, absorbed counts) are
     bit-exact.
 """
@@ -20,7 +23,7 @@ Tolerances (stated here, checked below):
 import numpy as np
 import pytest
 
-from conftest import say
+from conftest import check_records, fp32c_position_error_ok, say
 
 pytestmark = pytest.mark.gpu
 
@@ -113,15 +116,18 @@ def test_one_step_vs_oracle(oracle_lib, precision, mode, workload):
     dx = np.max(np.abs(x_g - x_o)) / cfg.R
     dv = np.max(np.abs(v_g - v_o)) / np.max(np.abs(v_o))
     say("  max |dx|/R", dx, "max |dv|/max|v|", dv)
-    assert dx <= (1e-11 if precision == "fp64" else 1e-8)
+    bound = None
+    if precision == "fp64":
+        assert dx <= 1e-11
+    else:
+        ok, worst, bound = fp32c_position_error_ok(oracle_lib, cfg, x0, x_g, x_o, fl_o)
+        say("  worst |dx_i| / bound_i", worst)
+        assert ok
     assert dv <= (1e-10 if precision == "fp64" else 1e-4)
-    tol_r = 1e-9 if precision == "fp64" else 1e-5
-    for key in ("q", "force_orbit", "force_collect", "force_dust"):
-        a = np.asarray(rec[key][0], dtype=np.float64)
-        b = np.asarray(rec_o[key], dtype=np.float64)
-        sc = max(np.max(np.abs(b)), 1e-300)
-        say("  record", key, "rel", np.max(np.abs(a - b)) / sc)
-        assert np.max(np.abs(a - b)) <= tol_r * sc, key
+    ratios = check_records(cfg, rec[0], rec_o, precision, x_o, bound)
+    say("  records, error / tolerance:", ratios)
+    for key, r in ratios.items():
+        assert r <= 1.0, key
 
 
 @pytest.mark.parametrize("workload", ["single_dust", "dust_chain"])
