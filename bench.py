@@ -134,6 +134,8 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
     b = engine.Batch(cfgs)
     for s_, (x0, v0) in enumerate(states):
         b.set_ions(s_, x0, v0)
+    chunk = b.tune(3)
+    log("sweep: autotuner chose %d source blocks per work-group" % chunk)
     b.run(3, engine.RUN_PROFILE_PAIR, records=False)
     b.pair_timing(reset=True)
     b.run(3, engine.RUN_PROFILE_PAIR, records=False)
@@ -168,6 +170,7 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
         "batch_pair_kernel_ms": kernel_ms,
         "batch_pair_kernel_tflops_alg": ppl * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12,
         "timed_steps": {"batched": steps, "sequential_per_sim": seq_steps},
+        "tuned_sym_chunk": chunk,
     }
 
 
@@ -187,6 +190,7 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=0,
                     help="also time the 64-simulation sweep, batched (this many steps) and sequential")
     ap.add_argument("--sweep-seq-steps", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -211,6 +215,10 @@ def main():
     eng = engine.Engine(cfg)
     eng.set_ions(x0, v0)
     mode, pairs_launch = eng.pair_info()
+    chunk = 0
+    if not args.no_tune and not args.pair_only:
+        chunk = eng.tune(2)
+        log("autotuner chose %d source blocks per work-group" % chunk)
 
     if args.pair_only:
         for _ in range(args.pair_only):
@@ -312,12 +320,15 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": SYM_TRAFFIC_2E20 if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
+            # profiled with the default schedule (16 source blocks per work-group at 2^20)
+            "traffic": SYM_TRAFFIC_2E20 if (mode == engine.PAIR_SYM and N == 2 ** 20 and chunk in (0, 16))
+            else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
             "flop_per_pair": FLOP_PER_PAIR,
             "pairs_per_launch": pairs_launch,
+            "sym_chunk": chunk,
         },
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,

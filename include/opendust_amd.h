@@ -192,6 +192,16 @@ int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t res
  * symmetric kernel.  No counterpart in the reference (measurement support). */
 int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
+/* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
+ * return *sym_chunk = 0 and change nothing): times the whole pair evaluation
+ * `reps` times for 1, 2, 4, 8, 16 and 32 source blocks per work-group on the
+ * current ions, keeps the fastest, and recomputes the ion-ion force of the
+ * current positions with it.  The number of blocks groups the partial sums,
+ * so results are reproducible for a given choice: *sym_chunk returns it, and
+ * od_config.sym_chunk fixes it.  Call after od_set_ions.  No counterpart in the
+ * reference (OpenMM picks its kernels internally). */
+int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk);
+
 /* Cutoff mode only: the cell list of the last force evaluation.  cell_of_ion[n]
  * (binned ions: their cell; ions left out: the number of cells), sorted_ions
  * (binned ions in cell order, ties by ion index), their count, and the grid
@@ -222,6 +232,32 @@ int od_trap_potential_grid(int32_t device, double R, double H, int32_t n_r, cons
 /* Sample coordinates of the Monte Carlo above (for checking the sampler):
  * k in [k0, k0 + n) -> xyz[n][3], units of r_D_e.  Pure host computation. */
 int od_trap_samples_host(double R, double H, uint64_t seed, int64_t k0, int64_t n, double* xyz);
+
+/* Axial ion-density profile of the context's current ions, computed on the
+ * device: replaces writing an XYZ frame and running tools/concentration.cpp
+ * (:32-101) on it.  n_nodes nodes z_grid[i] = -L/2 + i L/(n_nodes-1) (m);
+ * conc[i] in 1/m^3 from the tool's linear deposits (an ion in the first or
+ * last interval feeds only node 1 or n_nodes-2), divided by
+ * 0.5 (dz_left + dz_right) pi R^2 / N_s, end nodes extrapolated linearly.
+ * R, L (the domain radius and height) in m, N_s ions per superion.  Sums are
+ * the tool's sequential sums in ion order, bit for bit. */
+int od_density_profile(od_ctx* ctx, int32_t n_nodes, double R, double L, double N_s, double* z_grid, double* conc);
+
+/* The same for host positions z_nm[N] (nm), e.g. one frame of an XYZ file. */
+int od_density_profile_z(int32_t device, const double* z_nm, int64_t N, int32_t n_nodes, double R, double L,
+                         double N_s, double* z_grid, double* conc);
+
+/* ---- output files (host code, no device use) ----------------------------
+ * The reference's text formats, every number written as Python's "{}".format
+ * writes a float (shortest round-trip digits; fixed notation for decimal
+ * exponents -4..15, else scientific with a two-digit exponent).
+ * Replace the per-ion Python loops at opendust.py:1080-1088 (XYZ frame,
+ * appended: "N", "frame", then "Ar\tx\ty\tz" per ion), :1089-1099 (CSV frame
+ * via csv.writer: header x,y,z,density,mass, CRLF line ends) and :1100-1115
+ * (RESTART: "x\ty\tz\tvx\tvy\tvz" per ion).  pos, vel: [n][3] doubles. */
+int od_write_xyz(const char* path, const double* pos, int64_t n, int64_t frame);
+int od_write_csv(const char* path, const double* pos, int64_t n, double density, double mass);
+int od_write_restart(const char* path, const double* pos, const double* vel, int64_t n);
 
 /* ---- batched parameter sweeps ------------------------------------------
  * Many independent simulations advanced together on one device: one launch of
@@ -257,6 +293,11 @@ int od_batch_get_step_info(od_batch* b, int32_t sim, uint32_t* flags, uint8_t* f
  * OD_PAIR_ROWS in the first configuration).  Measurement support. */
 int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* launches, int64_t* pairs_per_launch,
                          int32_t reset);
+
+/* od_tune for a batch (all simulations share one schedule; the choice is
+ * fixed through the first configuration's sym_chunk).  Call after the
+ * od_batch_set_ions calls; the next od_batch_run recomputes the forces. */
+int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk);
 
 int od_batch_destroy(od_batch* b);
 

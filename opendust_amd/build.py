@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "od_engine.hip")
+SRCS = [os.path.join(HERE, "csrc", "od_engine.hip"), os.path.join(HERE, "csrc", "od_output.cpp")]
 INCLUDE = os.path.join(ROOT, "include")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libopendust_amd.so")
@@ -42,7 +42,7 @@ def build(verbose=False, save_temps=None):
     """Compile the engine; always recompiles (the 'does it build' check)."""
     os.makedirs(LIB_DIR, exist_ok=True)
     tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [hipcc()] + compile_flags() + [SRC, "-o", tmp]
+    cmd = [hipcc()] + compile_flags() + SRCS + ["-o", tmp]
     if save_temps:
         os.makedirs(save_temps, exist_ok=True)
         cmd.insert(1, "-save-temps=obj")

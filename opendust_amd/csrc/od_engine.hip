@@ -49,6 +49,9 @@ static int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
+// message setter for the library's other translation units
+int od_set_error_message(int code, const char* msg) { return set_err(code, "%s", msg); }
+
 #define HIPCHK(expr)                                                                                   \
   do {                                                                                                 \
     hipError_t e_ = (expr);                                                                            \
@@ -382,16 +385,21 @@ __device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& 
   bz = __shfl_xor(bz, mask, 64);
 }
 
+// Work-group (I, k) of the launch-order table wg_tab covers target block I and
+// source blocks I+1+k*ch ..; its partials live in slot wg_base[I] + k, so the
+// launch order (longest first) does not change any sum.
 // tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials.
 // Batched simulations: blk_end[I] (first block past I's simulation) and
 // blk_a[I] (its a) per block; null for one simulation (n_blocks, a).
 __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
-                                                    const int2* __restrict__ wg_tab, float a, float linv,
+                                                    const int2* __restrict__ wg_tab,
+                                                    const int* __restrict__ wg_base, float a, float linv,
                                                     const int* __restrict__ blk_end, const float* __restrict__ blk_a,
                                                     double* __restrict__ tp, float* __restrict__ sp) {
   __shared__ double red[4][3][64];
   const int2 ik = wg_tab[blockIdx.x];
   const int I = ik.x;
+  const size_t slot = (size_t)wg_base[I] + ik.y;
   const int J0 = I + 1 + ik.y * ch;
   const int J1 = min(J0 + ch, blk_end ? blk_end[I] : n_blocks);
   if (blk_a) a = blk_a[I];
@@ -409,7 +417,7 @@ __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ p
     sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
   }
   const float4* src = pos + (size_t)J0 * SYM_TB + lane;
-  float* spw = sp + (size_t)blockIdx.x * 3 * sp_len;
+  float* spw = sp + slot * 3 * sp_len;
   const auto seq = std::make_integer_sequence<int, 16>{};
   for (int b = 0; b < nsub; ++b) {
     const float4 p = src[(size_t)b * 64];
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ p
     }
     __syncthreads();
   }
-  double* tpw = tp + (size_t)blockIdx.x * 3 * SYM_TB;
+  double* tpw = tp + slot * 3 * SYM_TB;
 #pragma unroll
   for (int k = 0; k < SYM_MT; ++k) {
     const int t = wave * (64 * SYM_MT) + k * 64 + lane;
@@ -1148,20 +1156,47 @@ __global__ __launch_bounds__(256) void k_cell_gather(const uint32_t* __restrict_
   spos[p] = ((int)skey[p] < n_cells) ? pos[sval[p]] : Pos<T>::park();
 }
 
-// Forces on one wave's 64 targets (one cell, one chunk) from the 5x5x5 stencil;
-// sources are wave-uniform (scalar loads), pairs beyond the cutoff are masked.
-template <class T>
+// Truncated pair law with its parameters held in registers (SW: OpenMM switch).
+template <class T, int SW>
+struct CutLaw;
+template <int SW>
+struct CutLaw<float, SW> {
+  float a, linv, rc, rs, invw;
+  __device__ explicit CutLaw(const DevParams& P)
+      : a(P.a32), linv(P.linv32), rc(P.rc32), rs(P.rs32), invw(P.invw32) {}
+  __device__ __forceinline__ void operator()(const float4& i, const float4& j, Acc<float>& ax, Acc<float>& ay,
+                                             Acc<float>& az) const {
+    pair32_cut<SW != 0>(i.x, i.y, i.z, j.x, j.y, j.z, a, linv, rc, rs, invw, ax.a, ay.a, az.a);
+  }
+};
+template <int SW>
+struct CutLaw<double, SW> {
+  double rho0, rc, rs, invw;
+  __device__ explicit CutLaw(const DevParams& P) : rho0(P.rho0), rc(P.rc64), rs(P.rs64), invw(P.invw64) {}
+  __device__ __forceinline__ void operator()(const double4& i, const double4& j, Acc<double>& ax,
+                                             Acc<double>& ay, Acc<double>& az) const {
+    pair64_cut(i.x, i.y, i.z, j.x, j.y, j.z, rho0, SW != 0, rc, rs, invw, ax.a, ay.a, az.a);
+  }
+};
+
+// Forces on one wave's 64 targets (one cell, one chunk) from the 5x5x5 stencil.
+// The item index is made wave-uniform explicitly, so cell bounds and source
+// positions arrive through scalar loads and feed the VALU as SGPR operands;
+// the law's parameters live in registers; pairs beyond the cutoff are masked.
+template <class T, int SW>
 __global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__ P,
                                                    const typename Pos<T>::V* __restrict__ spos,
                                                    const uint32_t* __restrict__ sval,
                                                    const int* __restrict__ cell_start, const int* __restrict__ items,
                                                    const int* __restrict__ n_items, double* __restrict__ F) {
+  const CutLaw<T, SW> law(*P);
   const int lane = threadIdx.x & 63;
   const int np = P->n_pad;
   const int nx = P->nx, ny = P->ny, nz = P->nz;
+  const double scale = P->pair_scale;
   const int nit = n_items[0];
   const int nw = gridDim.x * 4;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nit; it += nw) {
+  for (int it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)); it < nit; it += nw) {
     const int packed = items[it];
     const int c = packed & 0xFFFF, k = packed >> 16;
     const int t0 = cell_start[c] + k * 64;
@@ -1177,7 +1212,15 @@ __global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__
         const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
         for (int jt = j0; jt < j1; jt += TJ) {
           const int je = min(jt + TJ, j1);
-          for (int j = jt; j < je; ++j) pair_acc<T>(*P, own, spos[j], ax, ay, az);
+          int j = jt;
+          for (; j + 4 <= je; j += 4) {
+            const typename Pos<T>::V p0 = spos[j], p1 = spos[j + 1], p2 = spos[j + 2], p3 = spos[j + 3];
+            law(own, p0, ax, ay, az);
+            law(own, p1, ax, ay, az);
+            law(own, p2, ax, ay, az);
+            law(own, p3, ax, ay, az);
+          }
+          for (; j < je; ++j) law(own, spos[j], ax, ay, az);
           ax.flush();
           ay.flush();
           az.flush();
@@ -1186,9 +1229,9 @@ __global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__
     }
     if (valid) {
       const int i = (int)sval[tp];
-      F[i] = ax.total() * P->pair_scale;
-      F[np + i] = ay.total() * P->pair_scale;
-      F[2 * np + i] = az.total() * P->pair_scale;
+      F[i] = ax.total() * scale;
+      F[np + i] = ay.total() * scale;
+      F[2 * np + i] = az.total() * scale;
     }
   }
 }
@@ -1308,6 +1351,7 @@ struct od_ctx {
   od_dust_record* rec = nullptr;
   int rec_cap = 1024;
   int64_t host_step = 0;
+  bool have_ions = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_steps = 0;
@@ -1467,12 +1511,73 @@ static void pair_geometry(int n_pad, int ib, int j_split, int* n_chunks, int* ch
   *chunk_len = len;
 }
 
-// workgroups of the triangular schedule: target block I, source blocks
-// I+1+k*ch .. I+(k+1)*ch (clipped), k = 0, 1, ...
-static int64_t sym_wg_count(int n_blocks, int ch) {
+// Work-groups of the triangular schedule: target block X, source blocks
+// X+1+k*ch .. X+(k+1)*ch clipped at be[X] (the first block past X's
+// simulation), k = 0, 1, ...
+static int64_t sym_wg_count(const std::vector<int>& be, int ch) {
   int64_t n = 0;
-  for (int I = 0; I < n_blocks; ++I) n += (n_blocks - 1 - I + ch - 1) / ch;
+  for (int X = 0; X < (int)be.size(); ++X) n += (be[X] - 1 - X + ch - 1) / ch;
   return n;
+}
+
+static double sym_bytes(const std::vector<int>& be, int ch) {
+  return (double)sym_wg_count(be, ch) * 3 * SYM_TB * (4.0 * ch + 8.0);
+}
+
+// (Re)builds a context's schedule for `ch` source blocks per work-group:
+// partial slots in row-major (X, k) order (wg_base), launch order longest
+// work-group first (ties in row-major order), so the short work-groups that
+// end each row fill the tail of the launch.  C: od_ctx or od_batch.
+template <class C>
+static int sym_build(C* c, const std::vector<int>& be, int ch) {
+  const int NB = (int)be.size();
+  const int64_t n_wg = sym_wg_count(be, ch);
+  if (n_wg > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "symmetric pair kernel: too many workgroups");
+  std::vector<int2> tab;
+  std::vector<int> base(NB + 1), len;
+  tab.reserve(n_wg);
+  len.reserve(n_wg);
+  for (int X = 0; X < NB; ++X) {
+    base[X] = (int)tab.size();
+    const int nk = (be[X] - 1 - X + ch - 1) / ch;
+    for (int k = 0; k < nk; ++k) {
+      tab.push_back(make_int2(X, k));
+      len.push_back(std::min(ch, be[X] - (X + 1 + k * ch)));
+    }
+  }
+  base[NB] = (int)tab.size();
+  std::vector<int> order(tab.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len[a] > len[b]; });
+  std::vector<int2> launch(tab.size());
+  for (size_t i = 0; i < order.size(); ++i) launch[i] = tab[order[i]];
+  void* old[] = {c->wg_tab, c->wg_base, c->sym_tp, c->sym_sp};
+  for (void* q : old)
+    if (q) hipFree(q);
+  c->wg_tab = nullptr;
+  c->wg_base = nullptr;
+  c->sym_tp = nullptr;
+  c->sym_sp = nullptr;
+  const size_t nw = std::max<size_t>(launch.size(), 1);
+  int rc = OD_OK;
+  if ((rc = dalloc(&c->wg_tab, nw * sizeof(int2))) || (rc = dalloc(&c->wg_base, base.size() * sizeof(int))) ||
+      (rc = dalloc(&c->sym_tp, nw * 3 * SYM_TB * 8)) || (rc = dalloc(&c->sym_sp, nw * 3 * ch * SYM_TB * 4)))
+    return rc;
+  if (!launch.empty()) HIPCHK(hipMemcpy(c->wg_tab, launch.data(), launch.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->wg_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->n_blocks = NB;
+  c->sym_ch = ch;
+  c->n_wg = (int)n_wg;
+  return OD_OK;
+}
+
+// default source blocks per work-group: the largest of 16, 8, .. that still
+// gives >= 4096 work-groups (8 per XCD CU set), or the configured value
+static int sym_default_chunk(const std::vector<int>& be, int configured) {
+  if (configured > 0) return configured;
+  int ch = 16;
+  while (ch > 1 && sym_wg_count(be, ch) < 4096) ch /= 2;
+  return ch;
 }
 
 static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
@@ -1505,34 +1610,12 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   const int T = c->n_pad / SYM_TB;
   const bool forced = cfg->pair_mode == OD_PAIR_SYM;
   if (T < 2 || (!forced && c->n_pad < (1 << 17))) return OD_OK;
-  int ch = cfg->sym_chunk > 0 ? cfg->sym_chunk : 16;
-  if (cfg->sym_chunk <= 0)
-    while (ch > 1 && sym_wg_count(T, ch) < 4096) ch /= 2;
-  const int64_t n_wg = sym_wg_count(T, ch);
-  const double bytes = (double)n_wg * 3 * SYM_TB * (4.0 * ch + 8.0);
-  if (!forced && bytes > 48e9) return OD_OK;
-  if (n_wg > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "symmetric pair kernel: too many workgroups");
-  std::vector<int2> tab;
-  std::vector<int> base(T + 1);
-  tab.reserve(n_wg);
-  for (int I = 0; I < T; ++I) {
-    base[I] = (int)tab.size();
-    const int nk = (T - 1 - I + ch - 1) / ch;
-    for (int k = 0; k < nk; ++k) tab.push_back(make_int2(I, k));
-  }
-  base[T] = (int)tab.size();
+  const std::vector<int> be(T, T);
+  const int ch = sym_default_chunk(be, cfg->sym_chunk);
+  if (!forced && sym_bytes(be, ch) > 48e9) return OD_OK;
   int rc = OD_OK;
-  if ((rc = dalloc(&c->wg_tab, tab.size() * sizeof(int2))) || (rc = dalloc(&c->wg_base, base.size() * sizeof(int))) ||
-      (rc = dalloc(&c->sym_tp, (size_t)n_wg * 3 * SYM_TB * 8)) ||
-      (rc = dalloc(&c->sym_sp, (size_t)n_wg * 3 * ch * SYM_TB * 4)) ||
-      (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)))
-    return rc;
-  HIPCHK(hipMemcpy(c->wg_tab, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->wg_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+  if ((rc = sym_build(c, be, ch)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8))) return rc;
   c->sym = 1;
-  c->n_blocks = T;
-  c->sym_ch = ch;
-  c->n_wg = (int)n_wg;
   return OD_OK;
 }
 
@@ -1637,8 +1720,8 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
     if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(256), 0, c->stream, (const float4*)pj, c->n_blocks,
-                       c->sym_ch, (const int2*)c->wg_tab, a, linv, (const int*)nullptr, (const float*)nullptr,
-                       c->sym_tp, c->sym_sp);
+                       c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, a, linv, (const int*)nullptr,
+                       (const float*)nullptr, c->sym_tp, c->sym_sp);
     if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
     hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
                        (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
@@ -1707,8 +1790,12 @@ static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const ui
   CHKRET(build_cells_t<T>(c, X, flags, pos));
   const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
   if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_pair_cell<T>, dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos, c->cval[0],
-                     c->cell_start, c->items, c->n_items, F);
+  if (c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff)
+    hipLaunchKernelGGL((k_pair_cell<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
+                       c->cval[0], c->cell_start, c->items, c->n_items, F);
+  else
+    hipLaunchKernelGGL((k_pair_cell<T, 0>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
+                       c->cval[0], c->cell_start, c->items, c->n_items, F);
   if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
   return OD_OK;
 }
@@ -1797,6 +1884,7 @@ extern "C" int od_set_ions(od_ctx* c, const double* x, const double* v, int64_t 
   CHKRET(launch_forces(c, c->X, c->pj, nullptr, c->F, false));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_ions = true;
   return OD_OK;
 }
 
@@ -2240,6 +2328,7 @@ struct od_batch {
   int64_t pair_launches = 0, pairs_per_launch = 0;
   // symmetric FP32 pair kernel over all simulations (1024-ion blocks)
   int sym = 0, sym_ch = 1, n_wg = 0, n_blocks = 0;
+  std::vector<int> blk_end_host;
   int2* wg_tab = nullptr;
   int *wg_base = nullptr, *blk_begin = nullptr, *blk_end = nullptr;
   float* blk_a = nullptr;
@@ -2378,21 +2467,6 @@ extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch**
         ba[X] = hp[s].a32;
         bs[X] = hp[s].pair_scale;
       }
-    auto count = [&](int ch) {
-      int64_t n = 0;
-      for (int X = 0; X < NB; ++X) n += (be[X] - 1 - X + ch - 1) / ch;
-      return n;
-    };
-    int ch = 16;
-    while (ch > 1 && count(ch) < 4096) ch /= 2;
-    std::vector<int2> tab;
-    std::vector<int> base(NB + 1);
-    for (int X = 0; X < NB; ++X) {
-      base[X] = (int)tab.size();
-      const int nk = (be[X] - 1 - X + ch - 1) / ch;
-      for (int k = 0; k < nk; ++k) tab.push_back(make_int2(X, k));
-    }
-    base[NB] = (int)tab.size();
     // ordered pairs of real ions the symmetric launch covers (different blocks)
     b->pairs_per_launch = 0;
     for (int s = 0; s < n_sims; ++s) {
@@ -2404,20 +2478,12 @@ extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch**
       }
       b->pairs_per_launch += n * (n - 1) - diag;
     }
-    b->n_blocks = NB;
-    b->sym_ch = ch;
-    b->n_wg = (int)tab.size();
-    const size_t n_wg = std::max<size_t>(tab.size(), 1);
-    if ((rc = dalloc(&b->wg_tab, n_wg * sizeof(int2))) || (rc = dalloc(&b->wg_base, base.size() * 4)) ||
-        (rc = dalloc(&b->blk_begin, NB * 4)) || (rc = dalloc(&b->blk_end, NB * 4)) ||
-        (rc = dalloc(&b->blk_a, NB * 4)) || (rc = dalloc(&b->blk_scale, NB * 8)) ||
-        (rc = dalloc(&b->sym_tp, n_wg * 3 * SYM_TB * 8)) || (rc = dalloc(&b->sym_sp, n_wg * 3 * ch * SYM_TB * 4)) ||
-        (rc = dalloc(&b->sym_dg, 3 * np * 8)))
+    b->blk_end_host = be;
+    if ((rc = sym_build(b, be, sym_default_chunk(be, cfgs[0].sym_chunk))) || (rc = dalloc(&b->blk_begin, NB * 4)) ||
+        (rc = dalloc(&b->blk_end, NB * 4)) || (rc = dalloc(&b->blk_a, NB * 4)) ||
+        (rc = dalloc(&b->blk_scale, NB * 8)) || (rc = dalloc(&b->sym_dg, 3 * np * 8)))
       return fail(rc);
-    if ((!tab.empty() &&
-         hipMemcpy(b->wg_tab, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(b->wg_base, base.data(), base.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->blk_begin, bb.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(b->blk_begin, bb.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->blk_end, be.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->blk_a, ba.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->blk_scale, bs.data(), NB * 8, hipMemcpyHostToDevice) != hipSuccess)
@@ -2475,21 +2541,21 @@ extern "C" int od_batch_get_ions(od_batch* b, int32_t sim, double* x, double* v)
 }
 
 template <class T>
-static void batch_pair(od_batch* b, bool timed) {
+static void batch_pair(od_batch* b, bool timed, double* Fout) {
   using PV = typename Pos<T>::V;
   if (b->sym) {
     const float linv = (float)(1.0 / 1.4426950408889634);
     if (timed) hipEventRecord(b->ev0, b->stream);
     if (b->n_wg > 0)
       hipLaunchKernelGGL(k_pair32_sym, dim3(b->n_wg), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_blocks,
-                         b->sym_ch, (const int2*)b->wg_tab, 0.0f, linv, (const int*)b->blk_end,
-                         (const float*)b->blk_a, b->sym_tp, b->sym_sp);
+                         b->sym_ch, (const int2*)b->wg_tab, (const int*)b->wg_base, 0.0f, linv,
+                         (const int*)b->blk_end, (const float*)b->blk_a, b->sym_tp, b->sym_sp);
     if (timed) hipEventRecord(b->ev1, b->stream);
     hipLaunchKernelGGL(k_pair32_diag, dim3(b->n_tot / 256), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_tot,
                        0.0f, linv, (const float*)b->blk_a, b->sym_dg);
     hipLaunchKernelGGL(k_sym_reduce, dim3(b->n_tot / 256), dim3(256), 0, b->stream, b->sym_dg, b->sym_tp,
                        b->sym_sp, b->wg_base, b->n_blocks, b->sym_ch, b->n_tot, 0.0, (const int*)b->blk_begin,
-                       (const double*)b->blk_scale, b->F);
+                       (const double*)b->blk_scale, Fout);
     return;
   }
   if (timed) hipEventRecord(b->ev0, b->stream);
@@ -2497,7 +2563,7 @@ static void batch_pair(od_batch* b, bool timed) {
                      b->part);
   if (timed) hipEventRecord(b->ev1, b->stream);
   hipLaunchKernelGGL(k_items_reduce, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->group_items,
-                     b->part, b->n_tot, b->F);
+                     b->part, b->n_tot, Fout);
 }
 
 template <class T>
@@ -2509,7 +2575,7 @@ static int batch_step_t(od_batch* b, bool timed) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, b->stream, b->block_cnt, nb, b->block_off, b->n_flag);
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, b->stream, b->flags, b->block_cnt, b->block_off, b->n_tot,
                      (const PV*)b->pi, b->slot_of, b->list_ion, (PV*)b->list_old);
-  batch_pair<T>(b, timed);
+  batch_pair<T>(b, timed, b->F);
   hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pi,
                      b->n_flag, (const PV*)b->list_old, b->flags, b->list_ion, (uint32_t)((1u << OD_MAX_DUST) - 1u),
                      b->rowD);
@@ -2535,7 +2601,7 @@ static int batch_forces_t(od_batch* b) {
   using PV = typename Pos<T>::V;
   hipLaunchKernelGGL(k_batch_load_pos<T>, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X,
                      (PV*)b->pi, (PV*)b->pj);
-  batch_pair<T>(b, false);
+  batch_pair<T>(b, false, b->F);
   return OD_OK;
 }
 
@@ -2627,4 +2693,197 @@ extern "C" int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* laun
     b->pair_launches = 0;
   }
   return OD_OK;
+}
+
+// Times the symmetric pair evaluation (kernel, diagonal, reduction) for each
+// candidate number of source blocks per work-group on the current positions
+// and keeps the fastest.  Results depend on that number (it groups the partial
+// sums), so runs are reproducible for a given choice, which is returned and
+// can be fixed through od_config.sym_chunk.
+static const int kTuneChunks[] = {1, 2, 4, 8, 16, 32};
+
+template <class C, class F>
+static int tune_chunks(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen) {
+  double best_ms = 1e300;
+  int best = c->sym_ch;
+  for (int ch : kTuneChunks) {
+    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > 48e9) continue;
+    CHKRET(sym_build(c, be, ch));
+    evaluate();  // warm-up
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) evaluate();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ch;
+    }
+  }
+  if (c->sym_ch != best) CHKRET(sym_build(c, be, best));
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+  *chosen = best;
+  return OD_OK;
+}
+
+extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (reps < 1 || reps > 100) return set_err(OD_ERR_ARG, "reps %d outside [1, 100]", reps);
+  int chosen = 0;
+  if (c->sym) {
+    if (!c->have_ions) return set_err(OD_ERR_STATE, "od_tune needs the ions (call od_set_ions first)");
+    HIPCHK(hipSetDevice(c->dev));
+    double* Fs = nullptr;
+    CHKRET(dalloc(&Fs, (size_t)3 * c->n_pad * 8));
+    const std::vector<int> be(c->n_blocks, c->n_blocks);
+    const int rc = tune_chunks(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen);
+    hipFree(Fs);
+    if (rc != OD_OK) return rc;
+    // the ion-ion force of the current positions, with the chosen schedule
+    CHKRET(load_positions(c, c->X, c->pi, c->pj));
+    CHKRET(launch_forces(c, c->X, c->pj, nullptr, c->F, false));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (sym_chunk) *sym_chunk = chosen;
+  return OD_OK;
+}
+
+extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk) {
+  if (!b) return set_err(OD_ERR_ARG, "null batch");
+  if (reps < 1 || reps > 100) return set_err(OD_ERR_ARG, "reps %d outside [1, 100]", reps);
+  int chosen = 0;
+  if (b->sym) {
+    HIPCHK(hipSetDevice(b->dev));
+    if (b->stale)
+      hipLaunchKernelGGL(k_batch_load_pos<float>, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X,
+                         (float4*)b->pi, (float4*)b->pj);
+    double* Fs = nullptr;
+    CHKRET(dalloc(&Fs, (size_t)3 * b->n_tot * 8));
+    const int rc = tune_chunks(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen);
+    hipFree(Fs);
+    if (rc != OD_OK) return rc;
+    b->stale = true;  // the next run recomputes the forces with the chosen schedule
+  }
+  if (sym_chunk) *sym_chunk = chosen;
+  return OD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Axial ion-density profile (tools/concentration.cpp:32-101).  One wave per
+// node walks the ions in index order; every lane computes its ion's deposit on
+// that node with the tool's expressions (FP64, no contraction), and the wave
+// adds the non-zero deposits in lane order, so each node's sum is the tool's
+// sequential sum bit for bit (an ion deposits at most once on a node, and
+// adding an exact zero changes nothing).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_density(const double* __restrict__ z_nm, int64_t N, int n, double L,
+                                                const double* __restrict__ zg, double* __restrict__ sums) {
+  const int node = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (int64_t j0 = 0; j0 < N; j0 += 64) {
+    const int64_t j = j0 + lane;
+    double v = 0.0;
+    if (j < N) {
+      const double z = z_nm[j] / 1e9;
+      const int index = (int)((0.5 * L + z) * (n - 1) / L);
+      if (index == 0 && z >= -0.5 * L) {
+        if (node == 1) v = 1.0 - (zg[1] - z) / (zg[1] - zg[0]);
+      } else if (index == n - 2) {
+        if (node == n - 2) v = 1.0 - (z - zg[n - 2]) / (zg[n - 1] - zg[n - 2]);
+      } else if (index < n - 2 && index > 0) {
+        if (node == index) v = 1.0 - (z - zg[index]) / (zg[index + 1] - zg[index]);
+        if (node == index + 1) v = 1.0 - (zg[index + 1] - z) / (zg[index + 1] - zg[index]);
+      }
+    }
+    uint64_t m = __ballot(v != 0.0);
+    while (m) {
+      const int l = __builtin_ctzll(m);
+      s = s + __shfl(v, l, 64);
+      m &= m - 1;
+    }
+  }
+  if (lane == 0) sums[node] = s;
+}
+
+static void density_finish(int n, double R, double L, double N_s, const std::vector<double>& sums, double* z_grid,
+                           double* conc) {
+  std::vector<double> zg(n);
+  for (int i = 0; i < n; i++) zg[i] = -L / 2.0 + i * L / (double)(n - 1);
+  std::vector<double> c(sums);
+  const double pi = 3.14159265358979323846;
+  for (int i = 1; i < n - 1; i++) c[i] /= 0.5 * ((zg[i + 1] - zg[i]) + (zg[i] - zg[i - 1])) * (pi * R * R) / N_s;
+  c[0] = 2 * c[1] - c[2];
+  c[n - 1] = 2 * c[n - 2] - c[n - 3];
+  for (int i = 0; i < n; i++) {
+    if (z_grid) z_grid[i] = zg[i];
+    if (conc) conc[i] = c[i];
+  }
+}
+
+static int density_run(hipStream_t st, const double* z_dev, int64_t N, int n, double R, double L, double N_s,
+                       double* z_grid, double* conc) {
+  std::vector<double> zg(n);
+  for (int i = 0; i < n; i++) zg[i] = -L / 2.0 + i * L / (double)(n - 1);
+  double *d_zg = nullptr, *d_s = nullptr;
+  int rc = OD_OK;
+  if ((rc = dalloc(&d_zg, (size_t)n * 8)) || (rc = dalloc(&d_s, (size_t)n * 8))) {
+    if (d_zg) hipFree(d_zg);
+    return rc;
+  }
+  std::vector<double> sums(n);
+  hipError_t e = hipMemcpyAsync(d_zg, zg.data(), (size_t)n * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_density, dim3(n), dim3(64), 0, st, z_dev, N, n, L, (const double*)d_zg, d_s);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(sums.data(), d_s, (size_t)n * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d_zg);
+  hipFree(d_s);
+  if (e != hipSuccess) return set_err(OD_ERR_HIP, "density profile: %s", hipGetErrorString(e));
+  density_finish(n, R, L, N_s, sums, z_grid, conc);
+  return OD_OK;
+}
+
+static int density_check(int n, double R, double L, double N_s) {
+  if (n < 3 || n > 65536) return set_err(OD_ERR_ARG, "density profile: %d nodes outside [3, 65536]", n);
+  if (!(R > 0.0) || !(L > 0.0) || !(N_s > 0.0) || !std::isfinite(R) || !std::isfinite(L) || !std::isfinite(N_s))
+    return set_err(OD_ERR_ARG, "density profile: R, L and N_s must be positive and finite");
+  return OD_OK;
+}
+
+extern "C" int od_density_profile(od_ctx* c, int32_t n_nodes, double R, double L, double N_s, double* z_grid,
+                                  double* conc) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  CHKRET(density_check(n_nodes, R, L, N_s));
+  if (!c->have_ions) return set_err(OD_ERR_STATE, "density profile: no ions yet (call od_set_ions first)");
+  HIPCHK(hipSetDevice(c->dev));
+  return density_run(c->stream, c->X + 2 * (size_t)c->n_pad, c->n, n_nodes, R, L, N_s, z_grid, conc);
+}
+
+extern "C" int od_density_profile_z(int32_t device, const double* z_nm, int64_t N, int32_t n_nodes, double R,
+                                    double L, double N_s, double* z_grid, double* conc) {
+  if (!z_nm && N > 0) return set_err(OD_ERR_ARG, "null positions");
+  if (N < 0) return set_err(OD_ERR_ARG, "negative ion count");
+  CHKRET(density_check(n_nodes, R, L, N_s));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return set_err(OD_ERR_ARG, "device %d not available", device);
+  HIPCHK(hipSetDevice(device));
+  double* d_z = nullptr;
+  CHKRET(dalloc(&d_z, (size_t)std::max<int64_t>(N, 1) * 8));
+  int rc = OD_OK;
+  if (N > 0 && hipMemcpy(d_z, z_nm, (size_t)N * 8, hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_err(OD_ERR_HIP, "density profile: upload failed");
+  if (rc == OD_OK) rc = density_run(nullptr, d_z, N, n_nodes, R, L, N_s, z_grid, conc);
+  hipFree(d_z);
+  return rc;
 }

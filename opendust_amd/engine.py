@@ -142,7 +142,8 @@ EXPORTS = [
     "od_trap_potential_grid", "od_trap_samples_host", "od_abi_sizes", "od_destroy",
     "od_last_error", "od_build_info",
     "od_batch_create", "od_batch_set_ions", "od_batch_get_ions", "od_batch_run", "od_batch_get_step_info",
-    "od_batch_pair_timing", "od_batch_destroy",
+    "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune",
+    "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
 ]
 
 
@@ -185,6 +186,15 @@ def load_library(path=LIB_PATH):
     lib.od_batch_get_step_info.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_uint32), P(ctypes.c_uint8)]
     lib.od_batch_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
     lib.od_batch_destroy.argtypes = [c_ctx]
+    lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32)]
+    lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
+    lib.od_write_csv.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+    lib.od_write_restart.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int64]
+    lib.od_density_profile.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       dp, dp]
+    lib.od_density_profile_z.argtypes = [ctypes.c_int32, dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, dp, dp]
+    lib.od_batch_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32)]
     lib.od_last_error.restype = ctypes.c_char_p
     lib.od_build_info.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -222,6 +232,46 @@ def trap_potential_grid(R_over_lambda, H_over_lambda, rho, h, n_mc, seed, device
                                       len(h), _dptr(h), int(n_mc), int(seed) & (2 ** 64 - 1), _dptr(out)),
            "od_trap_potential_grid")
     return out
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def write_xyz(path, pos, frame):
+    """Append one XYZ frame (opendust.py:1080-1088)."""
+    lib = load_library()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    _check(lib.od_write_xyz(_path(path), _dptr(pos), pos.shape[0], int(frame)), "od_write_xyz")
+
+
+def write_csv(path, pos, density, mass):
+    """Write one CSV frame (opendust.py:1089-1099)."""
+    lib = load_library()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    _check(lib.od_write_csv(_path(path), _dptr(pos), pos.shape[0], float(density), float(mass)), "od_write_csv")
+
+
+def write_restart(path, pos, vel):
+    """Write the RESTART file (opendust.py:1100-1115)."""
+    lib = load_library()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    if pos.shape != vel.shape:
+        raise ValueError("positions and velocities differ in shape")
+    _check(lib.od_write_restart(_path(path), _dptr(pos), _dptr(vel), pos.shape[0]), "od_write_restart")
+
+
+def density_profile_z(z_nm, R, L, N_s, n=100, device=0):
+    """Axial density profile (tools/concentration.cpp) of host positions
+    z_nm (nm) on the GPU: (z_grid[n] m, concentration[n] 1/m^3)."""
+    lib = load_library()
+    z = np.ascontiguousarray(z_nm, dtype=np.float64).ravel()
+    zg = np.empty(n)
+    c = np.empty(n)
+    _check(lib.od_density_profile_z(int(device), _dptr(z), z.size, int(n), float(R), float(L), float(N_s),
+                                    _dptr(zg), _dptr(c)), "od_density_profile_z")
+    return zg, c
 
 
 def trap_samples_host(R_over_lambda, H_over_lambda, seed, k0, n):
@@ -325,6 +375,22 @@ class Engine:
                "od_get_step_info")
         return flags, face, nr.value
 
+    def density_profile(self, R, L, N_s, n=100):
+        """Axial density profile of the current ions (tools/concentration.cpp):
+        (z_grid[n] m, concentration[n] 1/m^3); R, L in m."""
+        zg = np.empty(n)
+        c = np.empty(n)
+        _check(self.lib.od_density_profile(self._ctx, int(n), float(R), float(L), float(N_s), _dptr(zg), _dptr(c)),
+               "od_density_profile")
+        return zg, c
+
+    def tune(self, reps=3):
+        """Autotune the symmetric kernel's source blocks per work-group (0 when
+        the context does not run it); returns the choice."""
+        ch = ctypes.c_int32()
+        _check(self.lib.od_tune(self._ctx, int(reps), ctypes.byref(ch)), "od_tune")
+        return ch.value
+
     def pair_forces(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         f = np.empty_like(x)
@@ -388,6 +454,12 @@ class Batch:
                                                face.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
                "od_batch_get_step_info")
         return flags, face
+
+    def tune(self, reps=3):
+        """Autotune the batch's symmetric kernel (see Engine.tune)."""
+        ch = ctypes.c_int32()
+        _check(self.lib.od_batch_tune(self._b, int(reps), ctypes.byref(ch)), "od_batch_tune")
+        return ch.value
 
     def pair_timing(self, reset=False):
         ms = ctypes.c_double()

@@ -11,7 +11,6 @@ context and the per-step Python loop (opendust.py:1039-1384) with the HIP
 engine (opendust_amd/csrc/od_engine.hip) driven through the C ABI.
 """
 
-import csv
 import ctypes
 import secrets
 
@@ -19,7 +18,8 @@ import numpy as np
 from scipy import special
 
 from . import host
-from .engine import Config, Engine, RUN_GRAPH, trap_potential_grid, ABI_VERSION
+from .engine import (ABI_VERSION, RUN_GRAPH, Config, Engine, trap_potential_grid, write_csv, write_restart,
+                     write_xyz)
 
 
 class DustParticle:
@@ -434,25 +434,17 @@ class OpenDust:
             i = j + 1
 
     def _write_frame(self, i, pos):
+        """XYZ and CSV frames (opendust.py:1080-1099) through the native writer."""
         out = self.outputParameters
         idx = int(i / out.nFileOutput)
         if out.xyzOutputFileName != "":
-            with open(out.xyzOutputFileName, "a") as fh:
-                fh.write("{}\n".format(self.N))
-                fh.write("{}\n".format(idx))
-                fh.write("".join("Ar\t{}\t{}\t{}\n".format(a, b, c) for a, b, c in pos.tolist()))
+            write_xyz(out.xyzOutputFileName, pos, idx)
         if out.csvOutputFileName != "":
-            with open(out.csvOutputFileName + "{}.csv".format(idx), "w") as fh:
-                w = csv.writer(fh)
-                w.writerow(["x", "y", "z", "density", "mass"])
-                dens = self.plasmaParametersInSIUnits.n_inf * 1e-27
-                for a, b, c in pos.tolist():
-                    w.writerow([a, b, c, dens, self.N_s])
+            write_csv(out.csvOutputFileName + "{}.csv".format(idx), pos,
+                      self.plasmaParametersInSIUnits.n_inf * 1e-27, self.N_s)
 
     def _write_restart(self, pos, vel):
         """RESTART at the last step.  As in the reference (opendust.py:1100-1115)
         the positions are the ones of the last file-output frame, not of the
         last step; the velocities are current."""
-        with open(self.outputParameters.restartFileName, "w") as fh:
-            for (a, b, c), (u, v, w) in zip(pos.tolist(), vel.tolist()):
-                fh.write("{}\t{}\t{}\t{}\t{}\t{}\n".format(a, b, c, u, v, w))
+        write_restart(self.outputParameters.restartFileName, pos, vel)

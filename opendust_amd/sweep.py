@@ -72,21 +72,27 @@ def prepare_batch(openDusts, device=0, alphas=None, precision=None):
     return cfgs, states
 
 
-def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precision=None):
+def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precision=None, tune=False,
+                  sym_chunk=None):
     """Run every OpenDust object of `openDusts` for its own n steps, together.
     Simulations may differ in ion count, dust count and positions, plasma
     parameters and geometry; they share precision and device.  The batch steps
     until the longest simulation ends; the extra steps of shorter ones are
-    computed and discarded."""
+    computed and discarded.  tune=True lets the autotuner pick the pair
+    kernel's tile shape (od_batch_tune); the choice is returned, and results
+    are reproducible for it (pass it back as sym_chunk)."""
     if len(openDusts) == 0:
         return
     dev_str = str(deviceIndex).strip()
     device = int(dev_str.split(",")[0]) if dev_str else 0
     cfgs, states = prepare_batch(openDusts, device, alphas, precision)
+    if sym_chunk is not None:
+        cfgs[0].sym_chunk = int(sym_chunk)
     batch = Batch(cfgs)
     try:
         for s, (x0, v0) in enumerate(states):
             batch.set_ions(s, x0, v0)
+        chosen = batch.tune() if tune else None
         n_max = max(od.n for od in openDusts)
         flags = RUN_GRAPH if useGraph else 0
         i = 0
@@ -101,3 +107,4 @@ def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precis
             i += k
     finally:
         batch.close()
+    return chosen

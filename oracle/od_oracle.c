@@ -448,3 +448,36 @@ int orc_cell_of(const od_config* c, const double* x, int* dims) {
   }
   return ix + nx * (iy + ny * iz);
 }
+
+/* Axial ion-density profile of one frame (tools/concentration.cpp:32-101):
+ * n nodes z_grid[i] = -L/2 + i L/(n-1); every ion (z in nm, converted as the
+ * tool does, z/1e9) deposits linear (cloud-in-cell) weights on the two nodes
+ * around it, in ion order, except that an ion in the first interval only feeds
+ * node 1 and one in the last interval only node n-2; ions below -L/2 or at or
+ * beyond the last node are ignored.  Interior nodes are divided by
+ * 0.5 (dz_left + dz_right) pi R^2 / N_s, the end nodes extrapolated linearly.
+ * R, L in m; out: z_grid[n] (m), conc[n] (1/m^3). */
+void orc_density_profile(const double* z_nm, int64_t N, int n, double R, double L, double N_s, double* z_grid,
+                         double* conc) {
+  for (int i = 0; i < n; i++) {
+    z_grid[i] = -L / 2.0 + i * L / (double)(n - 1);
+    conc[i] = 0.0;
+  }
+  for (int64_t j = 0; j < N; j++) {
+    const double z = z_nm[j] / 1e9;
+    const int index = (int)((0.5 * L + z) * (n - 1) / L);
+    if (index == 0 && z >= -0.5 * L) {
+      conc[1] += 1.0 - (z_grid[1] - z) / (z_grid[1] - z_grid[0]);
+    } else if (index == n - 2) {
+      conc[n - 2] += 1.0 - (z - z_grid[n - 2]) / (z_grid[n - 1] - z_grid[n - 2]);
+    } else if (index < n - 2 && index > 0) {
+      conc[index] += 1.0 - (z - z_grid[index]) / (z_grid[index + 1] - z_grid[index]);
+      conc[index + 1] += 1.0 - (z_grid[index + 1] - z) / (z_grid[index + 1] - z_grid[index]);
+    }
+  }
+  const double pi = 3.14159265358979323846;
+  for (int i = 1; i < n - 1; i++)
+    conc[i] /= 0.5 * ((z_grid[i + 1] - z_grid[i]) + (z_grid[i] - z_grid[i - 1])) * (pi * R * R) / N_s;
+  conc[0] = 2 * conc[1] - conc[2];
+  conc[n - 1] = 2 * conc[n - 2] - conc[n - 3];
+}

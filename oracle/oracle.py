@@ -204,3 +204,17 @@ def cell_of(cfg, x):
     dims = (ctypes.c_int * 3)()
     cid = load().orc_cell_of(ctypes.byref(cfg), _d(x), dims)
     return cid, tuple(dims)
+
+
+def density_profile(z_nm, R, L, N_s, n=100):
+    """(z_grid[n] m, concentration[n] 1/m^3) of ions at axial positions z_nm (nm)."""
+    z = np.ascontiguousarray(z_nm, dtype=np.float64)
+    zg = np.empty(n)
+    c = np.empty(n)
+    lib = load()
+    lib.orc_density_profile.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+    lib.orc_density_profile.restype = None
+    lib.orc_density_profile(_d(z), len(z), int(n), float(R), float(L), float(N_s), _d(zg), _d(c))
+    return zg, c

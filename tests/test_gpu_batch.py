@@ -163,3 +163,31 @@ def test_simulate_batch_api_vs_oracle(oracle_lib):
             sc = np.max(np.abs(rec_o["force_orbit"][:, p]))
             assert np.max(np.abs(d.forceIonsOrbit - rec_o["force_orbit"][:, p])) <= 1e-9 * sc
             assert np.array_equal(d.forceDust.shape, (od.n, 3))
+
+
+def test_batch_tune_is_reproducible():
+    """The autotuner's choice fixes the schedule: a tuned batch and a batch
+    created with that sym_chunk give bitwise identical steps."""
+    from opendust_amd import engine
+
+    ods, cfgs, states = _batch_setup("fp32c")
+    say("batch tune", [c.n_ions for c in cfgs])
+    outs = []
+    chosen = None
+    for tune in (True, False):
+        if not tune:
+            cfgs[0].sym_chunk = chosen
+        b = engine.Batch(cfgs)
+        for s, (x0, v0) in enumerate(states):
+            b.set_ions(s, x0, v0)
+        if tune:
+            chosen = b.tune(2)
+            say("  chosen sym_chunk", chosen)
+            assert chosen in (1, 2, 4, 8, 16, 32)
+        rec = b.run(12, engine.RUN_GRAPH)
+        outs.append((rec, [b.get_ions(s) for s in range(len(cfgs))]))
+        b.close()
+    cfgs[0].sym_chunk = 0
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for (x, v), (x1, v1) in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(x, x1) and np.array_equal(v, v1)

@@ -169,6 +169,30 @@ def test_determinism_and_graph(oracle_lib, mode):
         assert np.array_equal(v, outs[0][2])
 
 
+def test_tune_is_reproducible():
+    """od_tune picks the symmetric kernel's source blocks per work-group; a
+    context created with that choice reproduces the tuned run bitwise."""
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=5000, precision="fp32c", craft=True, mode="sym")
+    say("tune fp32c sym N", cfg.n_ions)
+    outs = []
+    chosen = None
+    for tune in (True, False):
+        if not tune:
+            cfg.sym_chunk = chosen
+        e = eng.Engine(cfg)
+        e.set_ions(x0, v0)
+        if tune:
+            chosen = e.tune(2)
+            say("  chosen sym_chunk", chosen)
+            assert chosen in (1, 2, 4, 8, 16, 32)
+        rec = e.run(12, eng.RUN_GRAPH)
+        outs.append((rec, e.get_ions()))
+        e.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1][0], outs[1][1][0]) and np.array_equal(outs[0][1][1], outs[1][1][1])
+
+
 CUT_CASES = [("fp64", 1.5, None), ("fp32c", 1.5, None), ("fp32c", 3.2, 3.0)]
 
 

@@ -498,3 +498,55 @@ def test_trap_grid_center_against_quadrature(orc):
     d = np.linalg.norm(s, axis=1)
     sig = np.std(np.exp(-d) / (d + 0.001)) / np.sqrt(n_mc)
     assert abs(got - exact) <= 5 * sig
+
+
+# ---------------------------------------------------------------------------
+# axial density profile (tools/concentration.cpp:32-101)
+# ---------------------------------------------------------------------------
+def _profile_raw(oracle_lib, z_nm, R, L, N_s, n):
+    """Undo the interior normalisation to get the raw node deposits."""
+    zg, c = oracle_lib.density_profile(z_nm, R, L, N_s, n)
+    dz = L / (n - 1)
+    return zg, c, c * (dz * np.pi * R * R / N_s)
+
+
+def test_density_profile_single_ion_is_linear_interpolation(oracle_lib):
+    """An interior ion splits unit weight linearly between its two nodes;
+    nothing else is touched; nodes are -L/2 + i L/(n-1)."""
+    R, L, N_s, n = 1.5e-3, 6e-3, 64.0, 100
+    dz = L / (n - 1)
+    for idx, frac in [(1, 0.25), (37, 0.5), (50, 0.9), (97, 0.1)]:
+        z_m = -L / 2 + (idx + frac) * dz
+        zg, c, raw = _profile_raw(oracle_lib, [z_m * 1e9], R, L, N_s, n)
+        assert np.allclose(zg, -L / 2 + np.arange(n) * dz, rtol=0, atol=1e-18)
+        assert abs(raw[idx] - (1 - frac)) < 1e-9 and abs(raw[idx + 1] - frac) < 1e-9
+        others = np.delete(raw[1:-1], [idx - 1, idx])
+        assert np.all(others == 0.0)
+        # end nodes are the linear extrapolation of their neighbours
+        assert c[0] == 2 * c[1] - c[2] and c[-1] == 2 * c[-2] - c[-3]
+
+
+def test_density_profile_end_intervals_and_outside(oracle_lib):
+    """First interval: only node 1 gets its linear share; last interval: only
+    node n-2; ions below -L/2 or beyond the last node deposit nothing."""
+    R, L, N_s, n = 1.0e-3, 4e-3, 10.0, 50
+    dz = L / (n - 1)
+    zg, c, raw = _profile_raw(oracle_lib, [(-L / 2 + 0.3 * dz) * 1e9], R, L, N_s, n)
+    assert abs(raw[1] - 0.3) < 1e-9 and np.count_nonzero(raw[1:-1]) == 1
+    zg, c, raw = _profile_raw(oracle_lib, [(L / 2 - 0.3 * dz) * 1e9], R, L, N_s, n)
+    assert abs(raw[n - 2] - 0.3) < 1e-9 and np.count_nonzero(raw[1:-1]) == 1
+    zg, c, raw = _profile_raw(oracle_lib, [(-L / 2 - 2 * dz) * 1e9, (L / 2 + 0.5 * dz) * 1e9], R, L, N_s, n)
+    assert np.all(raw == 0.0) and np.all(c == 0.0)
+
+
+def test_density_profile_uniform_column(oracle_lib):
+    """Ions spread uniformly along z give the column density N N_s/(pi R^2 L)
+    at every interior node (up to the two end intervals' dropped shares) and,
+    extrapolated, at the ends."""
+    R, L, N_s, n = 2e-3, 8e-3, 50.0, 40
+    N = 400000
+    z = (-L / 2 + (np.arange(N) + 0.5) * L / N) * 1e9
+    zg, c = oracle_lib.density_profile(z, R, L, N_s, n)
+    target = N * N_s / (np.pi * R * R * L)
+    assert np.max(np.abs(c[2:-2] / target - 1)) < 1e-4
+    assert abs(c[0] / target - 1) < 1e-3 and abs(c[-1] / target - 1) < 1e-3
