@@ -166,6 +166,16 @@ int od_set_ions(od_ctx* ctx, const double* x, const double* v, int64_t n);
 /* Replaces context.getState(getPositions, getVelocities) (opendust.py:1074-1078, :1103-1105). */
 int od_get_ions(od_ctx* ctx, double* x, double* v);
 
+/* Replaces the context's parameters between runs, keeping the ion state on the
+ * device: dust positions, radii and charge methods, plasma constants, trap
+ * coefficients, reinjection tables and seed may change; the ion and dust
+ * counts, precision, device, pair kernel, cutoff and cylinder must not.  Dust
+ * charges keep their current values.  Serves dust-dynamics loops, which the
+ * reference runs as one full OpenDust(...).simulate() per dust step
+ * (examples/DustDynamics/TwoParticles/launch.py:113-160: new context, ions
+ * re-read from RESTART, trap refitted). */
+int od_update_config(od_ctx* ctx, const od_config* cfg);
+
 /* Replaces context.setParameter("q<p>", ...) (opendust.py:1151, :1202): q[n_dust], C. */
 int od_set_dust_charges(od_ctx* ctx, const double* q);
 int od_get_dust_charges(od_ctx* ctx, double* q);

@@ -1907,6 +1907,38 @@ extern "C" int od_get_ions(od_ctx* c, double* x, double* v) {
   return OD_OK;
 }
 
+// Replaces the context's parameters (dust positions, radii, charge methods,
+// plasma constants, trap coefficients, tables, seed) between runs while the
+// ion state stays on the device.  Sizes and geometry that buffers or the cell
+// list depend on must not change.  Dust charges keep their current values
+// (od_set_dust_charges sets them).
+extern "C" int od_update_config(od_ctx* c, const od_config* cfg) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  CHKRET(check_config(cfg));
+  const od_config& o = c->cfg;
+  if (cfg->n_ions != o.n_ions || cfg->n_dust != o.n_dust || cfg->precision != o.precision ||
+      cfg->device != o.device || cfg->pair_mode != o.pair_mode || cfg->j_split != o.j_split ||
+      cfg->cutoff != o.cutoff || cfg->switch_dist != o.switch_dist || cfg->R != o.R || cfg->H != o.H)
+    return set_err(OD_ERR_ARG,
+                   "od_update_config: ion count, dust count, precision, device, pair kernel, cutoff and the "
+                   "cylinder must stay the same");
+  HIPCHK(hipSetDevice(c->dev));
+  DevParams hp;
+  CHKRET(fill_params(cfg, c->n_pad, &hp));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->dP, &hp, sizeof(hp), hipMemcpyHostToDevice));
+  const int keep_chunk = c->cfg.sym_chunk;
+  c->cfg = *cfg;
+  c->cfg.sym_chunk = keep_chunk;
+  // captured steps hold kernel arguments derived from the old parameters
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+  return OD_OK;
+}
+
 extern "C" int od_set_dust_charges(od_ctx* c, const double* q) {
   if (!c || !q) return set_err(OD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));

@@ -144,6 +144,7 @@ EXPORTS = [
     "od_batch_create", "od_batch_set_ions", "od_batch_get_ions", "od_batch_run", "od_batch_get_step_info",
     "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune",
     "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
+    "od_update_config",
 ]
 
 
@@ -187,6 +188,7 @@ def load_library(path=LIB_PATH):
     lib.od_batch_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
     lib.od_batch_destroy.argtypes = [c_ctx]
     lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32)]
+    lib.od_update_config.argtypes = [c_ctx, P(Config)]
     lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
     lib.od_write_csv.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
     lib.od_write_restart.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int64]
@@ -315,6 +317,13 @@ class Engine:
         v = np.empty((self.n, 3))
         _check(self.lib.od_get_ions(self._ctx, _dptr(x), _dptr(v)), "od_get_ions")
         return x, v
+
+    def update_config(self, cfg):
+        """New parameters (dusts, plasma, trap, tables, seed) for the next runs;
+        the ions stay on the device (od_update_config)."""
+        cfg.struct_size = ctypes.sizeof(Config)
+        _check(self.lib.od_update_config(self._ctx, ctypes.byref(cfg)), "od_update_config")
+        self.cfg = cfg
 
     def set_dust_charges(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

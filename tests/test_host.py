@@ -244,3 +244,26 @@ def test_config_layout_and_constants():
     rho = np.hypot(x0[:, 0], x0[:, 1])
     assert np.all(rho <= cfg.R) and np.all(np.abs(x0[:, 2]) <= cfg.H / 2)
     assert np.all(np.linalg.norm(x0, axis=1) > cfg.dust_r[0])
+
+
+def test_dust_langevin_step_drag_gravity_and_noise():
+    """Without noise the update is the exact constant-acceleration step with
+    Epstein drag; the thermal kick has the stated variance."""
+    from opendust_amd.dynamics import dust_langevin_step
+
+    r_p, rho, P, v_T, dt = 4.445e-6, 1500.0, 18.13, 395.0, 5e-5
+    m_p = 4.0 / 3.0 * np.pi * r_p ** 3 * rho * 1.05
+    gamma = 8.0 / 3.0 * np.sqrt(2.0 * np.pi) * r_p ** 2 * P / v_T / m_p
+    x = np.array([1e-4, -2e-4, 3e-4])
+    v = np.array([1e-3, 0.0, -2e-3])
+    f = np.array([1e-13, 0.0, -5e-13])
+    xn, vn = dust_langevin_step(x, v, f, m_p, r_p, P, v_T, 0.0, dt)
+    a = (f - m_p * gamma * v) / m_p + np.array([0, 0, 9.8])
+    assert np.allclose(vn, v + a * dt, rtol=1e-14, atol=0)
+    assert np.allclose(xn, x + v * dt + 0.5 * a * dt ** 2, rtol=1e-14, atol=0)
+    rng = np.random.RandomState(0)
+    T = 300.0
+    dv = np.array([dust_langevin_step(x, np.zeros(3), np.zeros(3), m_p, r_p, P, v_T, T, dt, g=0.0, rng=rng)[1]
+                   for _ in range(4000)])
+    var_expect = 2 * 1.380649e-23 * T * m_p * gamma / dt * (dt / m_p) ** 2
+    assert abs(dv.var() / var_expect - 1) < 0.06
