@@ -706,7 +706,8 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pal
                                                const int* __restrict__ slot_of,
                                                const typename Pos<T>::V* __restrict__ list_old,
                                                typename Pos<T>::V* __restrict__ list_new, uint8_t* __restrict__ face_out,
-                                               const double* __restrict__ rowD, double* __restrict__ partials) {
+                                               const double* __restrict__ rowD, const double* __restrict__ G,
+                                               double* __restrict__ partials) {
   __shared__ double lds[4];
   const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
   const double* q = qall + P->q_off;
@@ -716,9 +717,10 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pal
   const bool valid = li < P->n;
   const typename Pos<T>::V own = valid ? pi[i] : Pos<T>::park();
   Acc<T> ax, ay, az;
-  // this simulation's flagged ions occupy list slots [l0, l1)
+  // this simulation's flagged ions occupy list slots [l0, l1); with G the
+  // flagged sources' share was computed in cell order (cutoff mode)
   const int l0 = block_off[P->blk0];
-  const int l1 = (P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0];
+  const int l1 = G ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
   for (int s0 = l0; s0 < l1; s0 += TJ) {
     const int s1 = min(s0 + TJ, l1);
     for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_old[s], ax, ay, az);
@@ -746,9 +748,15 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pal
       fz += rowD[3 * s + 2];
     } else {
       // flagged ions that only left the domain are reinjected below, whatever this kick gives
-      fx += F[i] + ax.total() * P->pair_scale;
-      fy += F[np + i] + ay.total() * P->pair_scale;
-      fz += F[2 * np + i] + az.total() * P->pair_scale;
+      if (G) {
+        fx += F[i] + G[i];
+        fy += F[np + i] + G[np + i];
+        fz += F[2 * np + i] + G[2 * np + i];
+      } else {
+        fx += F[i] + ax.total() * P->pair_scale;
+        fy += F[np + i] + ay.total() * P->pair_scale;
+        fz += F[2 * np + i] + az.total() * P->pair_scale;
+      }
     }
     vx = vx + P->half_dt * fx / P->m_s;
     vy = vy + P->half_dt * fy / P->m_s;
@@ -836,15 +844,17 @@ __global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ Pa
                                                 const typename Pos<T>::V* __restrict__ pj,
                                                 const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
                                                 const typename Pos<T>::V* __restrict__ list_new,
-                                                const int* __restrict__ slot_of, const double* __restrict__ rowF) {
+                                                const int* __restrict__ slot_of, const double* __restrict__ rowF,
+                                                int rows_only) {
   const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
   const int np = P->stride;
   const int i = blockIdx.x * 256 + threadIdx.x;
   const bool valid = i - P->ion_off < P->n;
   const typename Pos<T>::V own = valid ? pj[i] : Pos<T>::park();
   Acc<T> ax, ay, az;
+  // rows_only: the kept ions' share was added in cell order (cutoff mode)
   const int l0 = block_off[P->blk0];
-  const int l1 = (P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0];
+  const int l1 = rows_only ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
   for (int s0 = l0; s0 < l1; s0 += TJ) {
     const int s1 = min(s0 + TJ, l1);
     for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_new[s], ax, ay, az);
@@ -858,7 +868,7 @@ __global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ Pa
     F[i] = rowF[3 * s + 0];
     F[np + i] = rowF[3 * s + 1];
     F[2 * np + i] = rowF[3 * s + 2];
-  } else {
+  } else if (!rows_only) {
     F[i] += ax.total() * P->pair_scale;
     F[np + i] += ay.total() * P->pair_scale;
     F[2 * np + i] += az.total() * P->pair_scale;
@@ -1283,6 +1293,81 @@ __global__ __launch_bounds__(256) void k_rows_cell(const DevParams* __restrict__
   }
 }
 
+// Share of the listed (flagged) sources in the force on the binned (kept)
+// ions, cutoff mode.  Targets run in cell order, so a wave's 64 targets are
+// close together: a listed source is evaluated only if it lies within the
+// cutoff (plus a relative margin of 1e-5) of the wave's bounding box, decided
+// 64 sources at a time by a ballot; skipped sources are beyond the cutoff of
+// every target, where the law gives exactly zero.
+// out[ion] = (add ? out[ion] : 0) + sum * pair_scale.
+template <class T, int SW>
+__global__ __launch_bounds__(256) void k_corr_cell(const DevParams* __restrict__ P,
+                                                   const typename Pos<T>::V* __restrict__ spos,
+                                                   const uint32_t* __restrict__ sval,
+                                                   const int* __restrict__ cell_start, const int* __restrict__ n_flag,
+                                                   const typename Pos<T>::V* __restrict__ list, int add,
+                                                   double* __restrict__ out) {
+  using PV = typename Pos<T>::V;
+  const CutLaw<T, SW> law(*P);
+  const int np = P->n_pad;
+  const int nbin = cell_start[P->n_cells];
+  const int wave0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 256 + (threadIdx.x & ~63)));
+  if (wave0 >= nbin) return;
+  const int lane = threadIdx.x & 63;
+  const int p = wave0 + lane;
+  const bool valid = p < nbin;
+  const PV own = valid ? spos[p] : spos[wave0];
+  T lox = own.x, loy = own.y, loz = own.z, hix = own.x, hiy = own.y, hiz = own.z;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lox = min(lox, __shfl_xor(lox, o, 64));
+    loy = min(loy, __shfl_xor(loy, o, 64));
+    loz = min(loz, __shfl_xor(loz, o, 64));
+    hix = max(hix, __shfl_xor(hix, o, 64));
+    hiy = max(hiy, __shfl_xor(hiy, o, 64));
+    hiz = max(hiz, __shfl_xor(hiz, o, 64));
+  }
+  const T rcm = (sizeof(T) == 4 ? (T)P->rc32 : (T)P->rc64) * (T)1.00001;
+  const T rcm2 = rcm * rcm;
+  Acc<T> ax, ay, az;
+  int cnt = 0;
+  const int nf = n_flag[0];
+  for (int s0 = 0; s0 < nf; s0 += 64) {
+    const int s = s0 + lane;
+    const PV q = (s < nf) ? list[s] : Pos<T>::park();
+    const T dx = max(max(lox - q.x, q.x - hix), (T)0);
+    const T dy = max(max(loy - q.y, q.y - hiy), (T)0);
+    const T dz = max(max(loz - q.z, q.z - hiz), (T)0);
+    uint64_t m = __ballot(s < nf && dx * dx + dy * dy + dz * dz <= rcm2);
+    while (m) {
+      const int l = __builtin_ctzll(m);
+      m &= m - 1;
+      PV qs;
+      qs.x = __shfl(q.x, l, 64);
+      qs.y = __shfl(q.y, l, 64);
+      qs.z = __shfl(q.z, l, 64);
+      qs.w = 0;
+      law(own, qs, ax, ay, az);
+      if (++cnt == TJ) {
+        ax.flush();
+        ay.flush();
+        az.flush();
+        cnt = 0;
+      }
+    }
+  }
+  ax.flush();
+  ay.flush();
+  az.flush();
+  if (valid) {
+    const int i = (int)sval[p];
+    const double sc = P->pair_scale;
+    out[i] = (add ? out[i] : 0.0) + ax.total() * sc;
+    out[np + i] = (add ? out[np + i] : 0.0) + ay.total() * sc;
+    out[2 * np + i] = (add ? out[2 * np + i] : 0.0) + az.total() * sc;
+  }
+}
+
 // in-range and candidate pair counts over the current cell list (the same
 // distance test as k_pair_cell); integer atomics, so the totals are exact
 template <class T>
@@ -1364,7 +1449,8 @@ struct od_ctx {
   int* wg_base = nullptr;
   double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
   float* sym_sp = nullptr;
-  // cutoff mode: cell list
+  // cutoff mode: cell list; Gcut: flagged sources' share of the drift-state force
+  double* Gcut = nullptr;
   int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
   int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_count = nullptr, *cell_start = nullptr,
@@ -1477,7 +1563,7 @@ static void free_ctx(od_ctx* c) {
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
-                  c->rowD, c->sym_sp, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
+                  c->rowD, c->sym_sp, c->Gcut, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
                   c->rs_off, c->rs_total, c->cell_count, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
@@ -1599,7 +1685,8 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
       (rc = dalloc(&c->cell_start, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * 4)) ||
-      (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)))
+      (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)) ||
+      (rc = dalloc(&c->Gcut, 3 * np * 8)))
     return rc;
   return OD_OK;
 }
@@ -1805,6 +1892,18 @@ static int launch_forces(od_ctx* c, const double* X, const void* pos, const uint
                                     : launch_forces_t<double>(c, X, pos, flags, F, timed);
 }
 
+// flagged sources' share of the force on the binned ions, in cell order
+template <class T>
+static void launch_corr_cell(od_ctx* c, const typename Pos<T>::V* list, int add, double* out) {
+  using PV = typename Pos<T>::V;
+  if (c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff)
+    hipLaunchKernelGGL((k_corr_cell<T, 1>), dim3(c->nb), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
+                       c->cval[0], c->cell_start, c->n_flag, list, add, out);
+  else
+    hipLaunchKernelGGL((k_corr_cell<T, 0>), dim3(c->nb), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
+                       c->cval[0], c->cell_start, c->n_flag, list, add, out);
+}
+
 template <class T>
 static int launch_step_t(od_ctx* c, bool timed) {
   using PV = typename Pos<T>::V;
@@ -1828,10 +1927,12 @@ static int launch_step_t(od_ctx* c, bool timed) {
                          (const PV*)c->pi, c->n_flag, (const PV*)c->list_old, c->flags, c->list_ion, dust_bits,
                          c->rowD);
   }
+  if (c->cut) launch_corr_cell<T>(c, (const PV*)c->list_old, 0, c->Gcut);
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
                      c->X, c->V, c->F, c->q, c->st,
                      (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
-                     (PV*)c->list_new, c->face, c->rowD, c->partials);
+                     (PV*)c->list_new, c->face, c->rowD, c->cut ? (const double*)c->Gcut : (const double*)nullptr,
+                     c->partials);
   if (c->cut)
     hipLaunchKernelGGL(k_rows_cell<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, c->X, (const PV*)c->spos,
                        c->cell_start, c->n_flag, (const PV*)c->list_new, (const PV*)c->list_new, c->flags,
@@ -1839,8 +1940,10 @@ static int launch_step_t(od_ctx* c, bool timed) {
   else
     hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const int*)nullptr,
                        (const PV*)c->pj, c->n_flag, (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
+  if (c->cut) launch_corr_cell<T>(c, (const PV*)c->list_new, 1, c->F);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
-                     c->F, (const PV*)c->pj, c->flags, c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF);
+                     c->F, (const PV*)c->pj, c->flags, c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF,
+                     c->cut);
   hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, c->q, c->st, c->rec,
                      std::max(1, c->cfg.n_dust), 1);
   return OD_OK;
@@ -2613,11 +2716,12 @@ static int batch_step_t(od_batch* b, bool timed) {
                      b->rowD);
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->X, b->V,
                      b->F, b->q, b->st, (const PV*)b->pi, (PV*)b->pj, b->flags, b->n_flag, b->slot_of,
-                     (const PV*)b->list_old, (PV*)b->list_new, b->face, b->rowD, b->partials);
+                     (const PV*)b->list_old, (PV*)b->list_new, b->face, b->rowD, (const double*)nullptr,
+                     b->partials);
   hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pj,
                      b->n_flag, (const PV*)b->list_new, b->flags, b->list_ion, 0u, b->rowF);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->F,
-                     (const PV*)b->pj, b->flags, b->n_flag, (const PV*)b->list_new, b->slot_of, b->rowF);
+                     (const PV*)b->pj, b->flags, b->n_flag, (const PV*)b->list_new, b->slot_of, b->rowF, 0);
   hipLaunchKernelGGL(k_dust, dim3(b->n_sims), dim3(256), 0, b->stream, b->dP, b->partials, b->q, b->st, b->rec,
                      (int)OD_MAX_DUST, 0);
   hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->st);
