@@ -307,7 +307,8 @@ __global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__
 // are summed in a fixed order through LDS and stored as one partial per
 // (workgroup, source); target sums are Kahan-compensated across sub-blocks.
 // k_sym_reduce adds diagonal, target and source partials in a fixed order.
-// 13 + 3 + 3 VALU per unordered pair plus one 3-register rotate per 4 pairs.
+// 16 + 3 VALU per unordered pair (3 of them transcendental) plus one
+// 3-register rotate per 4 pairs.
 // ---------------------------------------------------------------------------
 constexpr int SYM_TB = 1024;  // ions per block of the triangular schedule
 constexpr int SYM_MT = 4;     // 64-target sub-blocks per wave
@@ -333,15 +334,52 @@ struct SymTargets {
 // sink an accumulator chain below it (either blows up register pressure).
 __device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
 
+// d = (source rotated by S lanes within each row of 16) - target, for the
+// wave's 4 target sub-blocks: the rotation rides on the subtraction as a DPP
+// modifier.  s_nop 1 keeps the two wait states a VALU write needs before a
+// DPP read of the same register.
+template <int S>
+__device__ __forceinline__ void sym_diffs(const SymTargets& T, float xj, float yj, float zj, float* dx, float* dy,
+                                          float* dz) {
+  if constexpr (S == 0) {
+#pragma unroll
+    for (int k = 0; k < SYM_MT; ++k) {
+      dx[k] = xj - T.x[k];
+      dy[k] = yj - T.y[k];
+      dz[k] = zj - T.z[k];
+    }
+  } else {
+    static_assert(SYM_MT == 4, "one asm block per four target sub-blocks");
+    asm("s_nop 1\n\t"
+        "v_sub_f32_dpp %0, %12, %15 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %1, %13, %16 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %2, %14, %17 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %3, %12, %18 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %4, %13, %19 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %5, %14, %20 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %6, %12, %21 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %7, %13, %22 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %8, %14, %23 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %9, %12, %25 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %10, %13, %26 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %11, %14, %27 row_ror:%c24 row_mask:0xf bank_mask:0xf"
+        : "=&v"(dx[0]), "=&v"(dy[0]), "=&v"(dz[0]), "=&v"(dx[1]), "=&v"(dy[1]), "=&v"(dz[1]), "=&v"(dx[2]),
+          "=&v"(dy[2]), "=&v"(dz[2]), "=&v"(dx[3]), "=&v"(dy[3]), "=&v"(dz[3])
+        : "v"(xj), "v"(yj), "v"(zj), "v"(T.x[0]), "v"(T.y[0]), "v"(T.z[0]), "v"(T.x[1]), "v"(T.y[1]),
+          "v"(T.z[1]), "v"(T.x[2]), "v"(T.y[2]), "v"(T.z[2]), "i"(S), "v"(T.x[3]), "v"(T.y[3]), "v"(T.z[3]));
+  }
+}
+
 template <int S>
 __device__ __forceinline__ void sym_step(SymTargets& T, float xj, float yj, float zj, float& bx, float& by,
                                          float& bz, float a, float linv) {
-  const float xs = dpp_ror<S>(xj), ys = dpp_ror<S>(yj), zs = dpp_ror<S>(zj);
+  float DX[SYM_MT], DY[SYM_MT], DZ[SYM_MT];
+  sym_diffs<S>(T, xj, yj, zj, DX, DY, DZ);
 #pragma unroll
   for (int k = 0; k < SYM_MT; ++k) {
-    const float dx = xs - T.x[k];
-    const float dy = ys - T.y[k];
-    const float dz = zs - T.z[k];
+    const float dx = DX[k];
+    const float dy = DY[k];
+    const float dz = DZ[k];
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32)));
     const float r = __builtin_amdgcn_sqrtf(r2);
     const float e = __builtin_amdgcn_exp2f(-r);
@@ -615,6 +653,51 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ block_cnt
     run += block_cnt[b];
   }
   if (t == 1023) n_flag[0] = s[1023];
+}
+
+// Exclusive scan of long int arrays in three launches: tile sums (1024
+// entries per block), a one-block scan of the tile sums (k_scan), and the
+// scan inside each tile plus its offset.  Integer sums: exact in any order.
+__global__ __launch_bounds__(256) void k_tile_sum(const int* __restrict__ in, int n, int* __restrict__ tsum) {
+  __shared__ int red[4];
+  const int base = blockIdx.x * 1024;
+  int v = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = base + r * 256 + (int)threadIdx.x;
+    if (i < n) v += in[i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) tsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_tile_scan(const int* __restrict__ in, int n, const int* __restrict__ toff,
+                                                   int* __restrict__ out) {
+  __shared__ int wsum[4];
+  const int i0 = blockIdx.x * 1024 + 4 * (int)threadIdx.x;
+  int v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : 0;
+  const int t = v[0] + v[1] + v[2] + v[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = t;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int run = toff[blockIdx.x] + incl - t;
+  for (int w = 0; w < wave; ++w) run += wsum[w];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (i0 + k < n) out[i0 + k] = run;
+    run += v[k];
+  }
 }
 
 // ordered list of flagged ions: slot = offset of the block + rank inside it
@@ -1134,16 +1217,22 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, int n, in
   if (i < n) a[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_cell_count(const uint32_t* __restrict__ key, int n, int n_cells,
-                                                    int* __restrict__ count) {
+// cell_start[c] = first position of the sorted keys with key >= c, c <= n_cells
+// (keys of ions left out are n_cells): position i fills the cells between
+// the previous key and its own.
+__global__ __launch_bounds__(256) void k_cell_starts(const uint32_t* __restrict__ skey, int n, int n_cells,
+                                                     int* __restrict__ cell_start) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n && (int)key[i] < n_cells) atomicAdd(&count[key[i]], 1);
+  if (i > n) return;
+  const int prev = (i == 0) ? -1 : min((int)skey[i - 1], n_cells);
+  const int cur = (i == n) ? n_cells : min((int)skey[i], n_cells);
+  for (int cc = prev + 1; cc <= cur; ++cc) cell_start[cc] = i;
 }
 
-__global__ __launch_bounds__(256) void k_cell_chunks(const int* __restrict__ count, int n_cells,
+__global__ __launch_bounds__(256) void k_cell_chunks(const int* __restrict__ cell_start, int n_cells,
                                                      int* __restrict__ chunks) {
   const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < n_cells) chunks[c] = (count[c] + 63) / 64;
+  if (c < n_cells) chunks[c] = (cell_start[c + 1] - cell_start[c] + 63) / 64;
 }
 
 // work items (cell, 64-target chunk), packed as cell | chunk << 16
@@ -1451,6 +1540,7 @@ struct od_ctx {
   float* sym_sp = nullptr;
   // cutoff mode: cell list; Gcut: flagged sources' share of the drift-state force
   double* Gcut = nullptr;
+  int *scan_tsum = nullptr, *scan_toff = nullptr;
   int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
   int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_count = nullptr, *cell_start = nullptr,
@@ -1563,7 +1653,7 @@ static void free_ctx(od_ctx* c) {
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
-                  c->rowD, c->sym_sp, c->Gcut, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
+                  c->rowD, c->sym_sp, c->Gcut, c->scan_tsum, c->scan_toff, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
                   c->rs_off, c->rs_total, c->cell_count, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
@@ -1688,6 +1778,8 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
       (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)) ||
       (rc = dalloc(&c->Gcut, 3 * np * 8)))
     return rc;
+  const size_t n_tiles = (std::max<size_t>((size_t)256 * c->rs_nblk, (size_t)c->n_cells + 1) + 1023) / 1024;
+  if ((rc = dalloc(&c->scan_tsum, n_tiles * 4)) || (rc = dalloc(&c->scan_toff, n_tiles * 4))) return rc;
   return OD_OK;
 }
 
@@ -1838,6 +1930,18 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
   return OD_OK;
 }
 
+// exclusive scan of n ints (in -> out, sum -> total) on the context's stream
+static void launch_scan(od_ctx* c, const int* in, int n, int* out, int* total) {
+  if (n <= 16384) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, in, n, out, total);
+    return;
+  }
+  const int nt = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_tile_sum, dim3(nt), dim3(256), 0, c->stream, in, n, c->scan_tsum);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int*)c->scan_tsum, nt, c->scan_toff, total);
+  hipLaunchKernelGGL(k_tile_scan, dim3(nt), dim3(256), 0, c->stream, in, n, (const int*)c->scan_toff, out);
+}
+
 // cell list of the ions at FP64 positions X (flagged ions left out when flags != null)
 template <class T>
 static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, const void* pos) {
@@ -1848,18 +1952,15 @@ static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, cons
     const int in = pass, out = 1 - pass, shift = 8 * pass;
     hipLaunchKernelGGL(k_rs_hist, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], np, shift, c->rs_nblk,
                        c->rs_hist);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->rs_hist, 256 * c->rs_nblk, c->rs_off,
-                       c->rs_total);
+    launch_scan(c, c->rs_hist, 256 * c->rs_nblk, c->rs_off, c->rs_total);
     hipLaunchKernelGGL(k_rs_scatter, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], c->cval[in], np, shift,
                        c->rs_nblk, c->rs_off, c->ckey[out], c->cval[out]);
   }
   const int gc = (nc + 256) / 256;
-  hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, c->stream, c->cell_count, nc + 1, 0);
-  hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, c->stream, c->ckey[0], np, nc, c->cell_count);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->cell_count, nc, c->cell_start,
-                     c->cell_start + nc);
-  hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_count, nc, c->cell_chunks);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->cell_chunks, nc, c->item_off, c->n_items);
+  hipLaunchKernelGGL(k_cell_starts, dim3(np / 256 + 1), dim3(256), 0, c->stream, c->ckey[0], np, nc,
+                     c->cell_start);
+  hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_start, nc, c->cell_chunks);
+  launch_scan(c, c->cell_chunks, nc, c->item_off, c->n_items);
   hipLaunchKernelGGL(k_cell_items, dim3(gc), dim3(256), 0, c->stream, c->cell_chunks, c->item_off, nc, c->items);
   hipLaunchKernelGGL(k_cell_gather<T>, dim3(nb), dim3(256), 0, c->stream, c->ckey[0], c->cval[0], np, nc,
                      (const PV*)pos, (PV*)c->spos);
