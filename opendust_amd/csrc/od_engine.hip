@@ -296,22 +296,23 @@ __global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__
 //
 // Ions are cut into blocks of SYM_TB.  A workgroup owns one target block I
 // and `ch` consecutive source blocks J > I (upper triangle); pairs inside a
-// block are done by k_pair32_diag.  Inside a workgroup each wave keeps 4
-// target sub-blocks of 64 ions in registers and streams 64-ion source
+// block are done by k_pair32_diag.  Inside a workgroup (two waves) each wave
+// keeps 8 target sub-blocks of 64 ions in registers and streams 64-ion source
 // sub-blocks.  Within a 16-lane row, step s pairs lane c's targets with the
 // source of lane (c - s) mod 16: the source position is read through a DPP
 // row-rotate folded into the subtraction, and the source's reaction
 // accumulator is rotated by one lane per step, so after 16 steps it is home.
 // Rows exchange sources (xor 16 / xor 32) between the four phases, which
-// covers all 64 x 64 pairs.  The four waves' reactions on a source sub-block
+// covers all 64 x 64 pairs.  The two waves' reactions on a source sub-block
 // are summed in a fixed order through LDS and stored as one partial per
 // (workgroup, source); target sums are Kahan-compensated across sub-blocks.
 // k_sym_reduce adds diagonal, target and source partials in a fixed order.
 // 16 + 3 VALU per unordered pair (3 of them transcendental) plus one
-// 3-register rotate per 4 pairs.
+// 3-register rotate per 8 pairs.
 // ---------------------------------------------------------------------------
 constexpr int SYM_TB = 1024;  // ions per block of the triangular schedule
-constexpr int SYM_MT = 4;     // 64-target sub-blocks per wave
+constexpr int SYM_MT = 8;     // 64-target sub-blocks per wave
+constexpr int SYM_NW = SYM_TB / (64 * SYM_MT);  // waves per workgroup
 
 template <int S>
 __device__ __forceinline__ float dpp_ror(float v) {
@@ -334,22 +335,21 @@ struct SymTargets {
 // sink an accumulator chain below it (either blows up register pressure).
 __device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
 
-// d = (source rotated by S lanes within each row of 16) - target, for the
-// wave's 4 target sub-blocks: the rotation rides on the subtraction as a DPP
+// d = (source rotated by S lanes within each row of 16) - target, for four of
+// the wave's target sub-blocks: the rotation rides on the subtraction as a DPP
 // modifier.  s_nop 1 keeps the two wait states a VALU write needs before a
 // DPP read of the same register.
 template <int S>
-__device__ __forceinline__ void sym_diffs(const SymTargets& T, float xj, float yj, float zj, float* dx, float* dy,
-                                          float* dz) {
+__device__ __forceinline__ void sym_diffs4(const float* tx, const float* ty, const float* tz, float xj, float yj,
+                                           float zj, float* dx, float* dy, float* dz) {
   if constexpr (S == 0) {
 #pragma unroll
-    for (int k = 0; k < SYM_MT; ++k) {
-      dx[k] = xj - T.x[k];
-      dy[k] = yj - T.y[k];
-      dz[k] = zj - T.z[k];
+    for (int k = 0; k < 4; ++k) {
+      dx[k] = xj - tx[k];
+      dy[k] = yj - ty[k];
+      dz[k] = zj - tz[k];
     }
   } else {
-    static_assert(SYM_MT == 4, "one asm block per four target sub-blocks");
     asm("s_nop 1\n\t"
         "v_sub_f32_dpp %0, %12, %15 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
         "v_sub_f32_dpp %1, %13, %16 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
@@ -365,9 +365,18 @@ __device__ __forceinline__ void sym_diffs(const SymTargets& T, float xj, float y
         "v_sub_f32_dpp %11, %14, %27 row_ror:%c24 row_mask:0xf bank_mask:0xf"
         : "=&v"(dx[0]), "=&v"(dy[0]), "=&v"(dz[0]), "=&v"(dx[1]), "=&v"(dy[1]), "=&v"(dz[1]), "=&v"(dx[2]),
           "=&v"(dy[2]), "=&v"(dz[2]), "=&v"(dx[3]), "=&v"(dy[3]), "=&v"(dz[3])
-        : "v"(xj), "v"(yj), "v"(zj), "v"(T.x[0]), "v"(T.y[0]), "v"(T.z[0]), "v"(T.x[1]), "v"(T.y[1]),
-          "v"(T.z[1]), "v"(T.x[2]), "v"(T.y[2]), "v"(T.z[2]), "i"(S), "v"(T.x[3]), "v"(T.y[3]), "v"(T.z[3]));
+        : "v"(xj), "v"(yj), "v"(zj), "v"(tx[0]), "v"(ty[0]), "v"(tz[0]), "v"(tx[1]), "v"(ty[1]),
+          "v"(tz[1]), "v"(tx[2]), "v"(ty[2]), "v"(tz[2]), "i"(S), "v"(tx[3]), "v"(ty[3]), "v"(tz[3]));
   }
+}
+
+template <int S>
+__device__ __forceinline__ void sym_diffs(const SymTargets& T, float xj, float yj, float zj, float* dx, float* dy,
+                                          float* dz) {
+  static_assert(SYM_MT % 4 == 0, "targets per wave in groups of four");
+#pragma unroll
+  for (int g = 0; g < SYM_MT; g += 4)
+    sym_diffs4<S>(T.x + g, T.y + g, T.z + g, xj, yj, zj, dx + g, dy + g, dz + g);
 }
 
 template <int S>
@@ -429,12 +438,12 @@ __device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& 
 // tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials.
 // Batched simulations: blk_end[I] (first block past I's simulation) and
 // blk_a[I] (its a) per block; null for one simulation (n_blocks, a).
-__global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
+__global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
                                                     const int2* __restrict__ wg_tab,
                                                     const int* __restrict__ wg_base, float a, float linv,
                                                     const int* __restrict__ blk_end, const float* __restrict__ blk_a,
                                                     double* __restrict__ tp, float* __restrict__ sp) {
-  __shared__ double red[4][3][64];
+  __shared__ double red[SYM_NW][3][64];
   const int2 ik = wg_tab[blockIdx.x];
   const int I = ik.x;
   const size_t slot = (size_t)wg_base[I] + ik.y;
@@ -483,9 +492,11 @@ __global__ __launch_bounds__(256) void k_pair32_sym(const float4* __restrict__ p
     red[wave][1][lane] = by;
     red[wave][2][lane] = bz;
     __syncthreads();
-    if (threadIdx.x < 192) {
-      const int c = threadIdx.x >> 6, l = threadIdx.x & 63;
-      const double v = ((red[0][c][l] + red[1][c][l]) + red[2][c][l]) + red[3][c][l];
+    for (int e = threadIdx.x; e < 192; e += 64 * SYM_NW) {
+      const int c = e >> 6, l = e & 63;
+      double v = red[0][c][l];
+#pragma unroll
+      for (int w = 1; w < SYM_NW; ++w) v += red[w][c][l];
       spw[c * sp_len + (size_t)b * 64 + l] = (float)v;
     }
     __syncthreads();
@@ -1898,7 +1909,7 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
   if (c->sym && pi == pj) {
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
     if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(256), 0, c->stream, (const float4*)pj, c->n_blocks,
+    hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(64 * SYM_NW), 0, c->stream, (const float4*)pj, c->n_blocks,
                        c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, a, linv, (const int*)nullptr,
                        (const float*)nullptr, c->sym_tp, c->sym_sp);
     if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -2783,7 +2794,7 @@ static void batch_pair(od_batch* b, bool timed, double* Fout) {
     const float linv = (float)(1.0 / 1.4426950408889634);
     if (timed) hipEventRecord(b->ev0, b->stream);
     if (b->n_wg > 0)
-      hipLaunchKernelGGL(k_pair32_sym, dim3(b->n_wg), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_blocks,
+      hipLaunchKernelGGL(k_pair32_sym, dim3(b->n_wg), dim3(64 * SYM_NW), 0, b->stream, (const float4*)b->pj, b->n_blocks,
                          b->sym_ch, (const int2*)b->wg_tab, (const int*)b->wg_base, 0.0f, linv,
                          (const int*)b->blk_end, (const float*)b->blk_a, b->sym_tp, b->sym_sp);
     if (timed) hipEventRecord(b->ev1, b->stream);
