@@ -30,10 +30,10 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector with packed ops; also matrix), TFLOP/s
 FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
-# HBM bytes per k_pair32_sym launch at 2^20 ions (default source chunk), from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy's readings (x2.000,
-# x1.000): profiles/k_pair32_sym_2e20_pmc.md
-SYM_TRAFFIC_2E20 = 13.405e9
+# HBM bytes per k_pair32_sym launch at 2^20 ions for a given number of source
+# blocks per work-group, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a
+# 1 GiB copy's readings (x2.000, x1.000): profiles/k_pair32_sym_2e20_pmc.md
+SYM_TRAFFIC_2E20 = {16: 13.873e9, 1: 27.430e9}
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
 
 
@@ -185,11 +185,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pair-only", type=int, default=0)
     ap.add_argument("--pair-mode", choices=["sym", "rows", "auto"], default="sym")
-    ap.add_argument("--cutoff-steps", type=int, default=0,
+    ap.add_argument("--cutoff-steps", type=int, default=3,
                     help="also time the cutoff workload (2^22 ions, R=10, H=20 r_D_e, r_c=2 r_D_e)")
-    ap.add_argument("--sweep-steps", type=int, default=0,
+    ap.add_argument("--sweep-steps", type=int, default=10,
                     help="also time the 64-simulation sweep, batched (this many steps) and sequential")
-    ap.add_argument("--sweep-seq-steps", type=int, default=20)
+    ap.add_argument("--sweep-seq-steps", type=int, default=5)
+    ap.add_argument("--sym-chunk", type=int, default=0,
+                    help="source blocks per work-group of the symmetric kernel (0: autotune, or the default "
+                         "with --no-tune)")
     ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
     args = ap.parse_args()
 
@@ -210,13 +213,14 @@ def main():
                                trapSeed=2024)
     cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=7, precision="fp32c")
     cfg.pair_mode = {"rows": engine.PAIR_ROWS, "sym": engine.PAIR_SYM, "auto": engine.PAIR_AUTO}[args.pair_mode]
+    cfg.sym_chunk = args.sym_chunk
     log("setup done in %.1f s (p_t=%.4f p_s=%.4f)" % (time.time() - t0, od.p_t, od.p_s))
 
     eng = engine.Engine(cfg)
     eng.set_ions(x0, v0)
     mode, pairs_launch = eng.pair_info()
-    chunk = 0
-    if not args.no_tune and not args.pair_only:
+    chunk = args.sym_chunk
+    if not args.no_tune and not args.pair_only and not args.sym_chunk:
         chunk = eng.tune(2)
         log("autotuner chose %d source blocks per work-group" % chunk)
 
@@ -320,9 +324,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            # profiled with the default schedule (16 source blocks per work-group at 2^20)
-            "traffic": SYM_TRAFFIC_2E20 if (mode == engine.PAIR_SYM and N == 2 ** 20 and chunk in (0, 16))
-            else None,
+            # profiled per schedule (the untuned default is 16 source blocks per work-group at 2^20)
+            "traffic": SYM_TRAFFIC_2E20.get(chunk or 16) if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
