@@ -73,14 +73,15 @@ def prepare_batch(openDusts, device=0, alphas=None, precision=None):
 
 
 def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precision=None, tune=False,
-                  sym_chunk=None):
+                  sym_chunk=None, progress=None):
     """Run every OpenDust object of `openDusts` for its own n steps, together.
     Simulations may differ in ion count, dust count and positions, plasma
     parameters and geometry; they share precision and device.  The batch steps
     until the longest simulation ends; the extra steps of shorter ones are
     computed and discarded.  tune=True lets the autotuner pick the pair
     kernel's tile shape (od_batch_tune); the choice is returned, and results
-    are reproducible for it (pass it back as sym_chunk)."""
+    are reproducible for it (pass it back as sym_chunk).  progress(done, total)
+    is called after every chunk of steps."""
     if len(openDusts) == 0:
         return
     dev_str = str(deviceIndex).strip()
@@ -105,6 +106,8 @@ def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precis
                 kk = min(k, od.n - i)
                 od._record(i, rec[:kk, s, :od.numberOfDustParticles])
             i += k
+            if progress is not None:
+                progress(i, n_max)
     finally:
         batch.close()
     return chosen

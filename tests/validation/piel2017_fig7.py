@@ -68,7 +68,8 @@ def main():
         ods = build(points)
     np.random.seed(2017)
     t0 = time.time()
-    sweep.simulateBatch(ods, "0", tune=True)
+    sweep.simulateBatch(ods, "0", tune=True,
+                        progress=lambda i, n: print("  step %d of %d, %.0f s" % (i, n, time.time() - t0), flush=True))
     wall = time.time() - t0
     rows = []
     for od, p in zip(ods, points):
