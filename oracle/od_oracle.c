@@ -25,9 +25,24 @@
  * reinjection writes x.
  *
  * Randomness follows the engine's counter-based specification (Philox4x32-10,
- * Salmon et al. SC'11), restated here: counter {ion, step, sim=0,
+ * Salmon et al. SC'11), restated here: counter {ion, step, sim_id,
  * stream | sub<<8}, key = seed; u53(a,b) = ((a>>5)*2^26 + (b>>6)) 2^-53;
  * Box-Muller on (u53(w0,w1) + 2^-53, u53(w2,w3)).
+ *
+ * Also: the trap Monte Carlo (opendust.py:499-523), the cell binning
+ * specification of the cutoff mode (include/opendust_amd.h), and the axial
+ * density profile of tools/concentration.cpp:32-101.
+ *
+ * How each part is pinned (tests/test_oracle_pinning.py, tests/test_host.py):
+ * Philox by published known-answer vectors and rocRAND; pair, switched-cutoff,
+ * dust, trap and field forces as minus the gradient of the reference's
+ * energies (finite differences); KDK, flags, counts, momenta and the step
+ * ordering by closed forms on crafted states; reinjection by its sampling
+ * distributions; charge updates by textbook limits; dust forces by Coulomb
+ * gradients; the trap grid by quadrature; the density profile by linear
+ * interpolation identities.  OpenMM's own kernels and random streams are not
+ * reproducible without OpenMM: that level is parity unpinned (the end-to-end
+ * statistics of the reference's example runs are the check there).
  */
 #include <math.h>
 #include <stdint.h>
