@@ -1223,11 +1223,6 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, int n, int v) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) a[i] = v;
-}
-
 // cell_start[c] = first position of the sorted keys with key >= c, c <= n_cells
 // (keys of ions left out are n_cells): position i fills the cells between
 // the previous key and its own.
@@ -1554,7 +1549,7 @@ struct od_ctx {
   int *scan_tsum = nullptr, *scan_toff = nullptr;
   int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
-  int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_count = nullptr, *cell_start = nullptr,
+  int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_start = nullptr,
       *cell_chunks = nullptr, *item_off = nullptr, *items = nullptr, *n_items = nullptr;
   void* spos = nullptr;
 };
@@ -1665,7 +1660,7 @@ static void free_ctx(od_ctx* c) {
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
                   c->rowD, c->sym_sp, c->Gcut, c->scan_tsum, c->scan_toff, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
-                  c->rs_off, c->rs_total, c->cell_count, c->cell_start, c->cell_chunks, c->item_off, c->items,
+                  c->rs_off, c->rs_total, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -1782,7 +1777,6 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
   if ((rc = dalloc(&c->ckey[0], np * 4)) || (rc = dalloc(&c->ckey[1], np * 4)) || (rc = dalloc(&c->cval[0], np * 4)) ||
       (rc = dalloc(&c->cval[1], np * 4)) || (rc = dalloc(&c->rs_hist, (size_t)256 * c->rs_nblk * 4)) ||
       (rc = dalloc(&c->rs_off, (size_t)256 * c->rs_nblk * 4)) || (rc = dalloc(&c->rs_total, 4)) ||
-      (rc = dalloc(&c->cell_count, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->cell_start, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * 4)) ||

@@ -298,3 +298,32 @@ def test_simulate_api_small(tmp_path):
     assert lines[0] == "4096" and len(lines) == 3 * (4096 + 2)
     rs = open(tmp_path / "RESTART").read().splitlines()
     assert len(rs) == 4096 and len(rs[0].split("\t")) == 6
+
+
+@pytest.mark.parametrize("N,n_dust", [(1, 1), (7, 0), (1500, 0)])
+def test_edge_sizes_and_no_dust_vs_oracle(oracle_lib, N, n_dust):
+    """A single ion, a handful of ions, and a cylinder without dust: the same
+    kernels and padding, checked against the oracle for three steps (FP64)."""
+    import ctypes
+
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=max(N, 64), precision="fp64", craft=False)
+    cfg = eng.Config.from_buffer_copy(cfg)
+    cfg.n_ions = N
+    x0, v0 = x0[:N].copy(), v0[:N].copy()
+    if n_dust == 0:
+        cfg.n_dust = 0
+    say("edge case N", N, "dusts", cfg.n_dust)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    rec = e.run(3)
+    x_g, v_g = e.get_ions()
+    e.close()
+    q0 = np.array([cfg.dust_q0[p] for p in range(cfg.n_dust)])
+    x_o, v_o, q_o, rec_o = oracle_lib.run(cfg, x0, v0, q0, 0, 3)
+    assert rec.shape == (3, cfg.n_dust)
+    assert np.max(np.abs(x_g - x_o)) <= 1e-10 * cfg.R
+    assert np.max(np.abs(v_g - v_o)) <= 1e-9 * max(np.max(np.abs(v_o)), 1e-30)
+    if cfg.n_dust:
+        assert np.array_equal(rec["absorbed"], rec_o["absorbed"])
+        assert np.max(np.abs(rec["q"] - rec_o["q"]) / np.abs(rec_o["q"])) <= 1e-12
