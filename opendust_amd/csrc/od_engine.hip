@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -325,9 +326,17 @@ __device__ __forceinline__ float dpp_ror(float v) {
   }
 }
 
-struct SymTargets {
-  float x[SYM_MT], y[SYM_MT], z[SYM_MT];
-  float ax[SYM_MT], ay[SYM_MT], az[SYM_MT];
+template <int MT>
+struct SymT {
+  float x[MT], y[MT], z[MT];
+  float ax[MT], ay[MT], az[MT];
+};
+using SymTargets = SymT<SYM_MT>;
+
+// pair-law parameters of the symmetric kernels (FP32, units r_D_e / log2 e)
+struct SymLaw {
+  float a, linv;
+  float rc, rs, invw;  // truncation (cell-list kernel)
 };
 
 // Empty asm with a read-write register operand: the value must exist at this
@@ -370,31 +379,41 @@ __device__ __forceinline__ void sym_diffs4(const float* tx, const float* ty, con
   }
 }
 
-template <int S>
-__device__ __forceinline__ void sym_diffs(const SymTargets& T, float xj, float yj, float zj, float* dx, float* dy,
+template <int S, int MT>
+__device__ __forceinline__ void sym_diffs(const SymT<MT>& T, float xj, float yj, float zj, float* dx, float* dy,
                                           float* dz) {
-  static_assert(SYM_MT % 4 == 0, "targets per wave in groups of four");
+  static_assert(MT % 4 == 0, "targets per wave in groups of four");
 #pragma unroll
-  for (int g = 0; g < SYM_MT; g += 4)
+  for (int g = 0; g < MT; g += 4)
     sym_diffs4<S>(T.x + g, T.y + g, T.z + g, xj, yj, zj, dx + g, dy + g, dz + g);
 }
 
-template <int S>
-__device__ __forceinline__ void sym_step(SymTargets& T, float xj, float yj, float zj, float& bx, float& by,
-                                         float& bz, float a, float linv) {
-  float DX[SYM_MT], DY[SYM_MT], DZ[SYM_MT];
+// CUT: 0 all pairs, 1 hard truncation at rc, 2 OpenMM switch from rs to rc
+template <int S, int MT, int CUT>
+__device__ __forceinline__ void sym_step(SymT<MT>& T, float xj, float yj, float zj, float& bx, float& by,
+                                         float& bz, const SymLaw& L) {
+  float DX[MT], DY[MT], DZ[MT];
   sym_diffs<S>(T, xj, yj, zj, DX, DY, DZ);
 #pragma unroll
-  for (int k = 0; k < SYM_MT; ++k) {
+  for (int k = 0; k < MT; ++k) {
     const float dx = DX[k];
     const float dy = DY[k];
     const float dz = DZ[k];
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32)));
     const float r = __builtin_amdgcn_sqrtf(r2);
     const float e = __builtin_amdgcn_exp2f(-r);
-    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(a, r, r2));
-    const float t = __builtin_fmaf(r, w, linv);
-    const float h = e * w * t;
+    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(L.a, r, r2));
+    const float t = __builtin_fmaf(r, w, L.linv);
+    float h = e * w * t;
+    if constexpr (CUT == 2) {
+      const float x = __builtin_fminf(__builtin_fmaxf((r - L.rs) * L.invw, 0.0f), 1.0f);
+      const float x2 = x * x;
+      const float Sx = __builtin_fmaf(x2 * x, __builtin_fmaf(x, __builtin_fmaf(x, -6.0f, 15.0f), -10.0f), 1.0f);
+      const float omx = 1.0f - x;
+      const float dS = -30.0f * x2 * omx * omx * L.invw;
+      h = __builtin_fmaf(h, Sx, -(e * w * dS));
+    }
+    if constexpr (CUT >= 1) h = (r < L.rc) ? h : 0.0f;
     T.ax[k] = __builtin_fmaf(h, dx, T.ax[k]);
     T.ay[k] = __builtin_fmaf(h, dy, T.ay[k]);
     T.az[k] = __builtin_fmaf(h, dz, T.az[k]);
@@ -409,17 +428,11 @@ __device__ __forceinline__ void sym_step(SymTargets& T, float xj, float yj, floa
   pin(by);
   pin(bz);
 #pragma unroll
-  for (int k = 0; k < SYM_MT; ++k) {
+  for (int k = 0; k < MT; ++k) {
     pin(T.ax[k]);
     pin(T.ay[k]);
     pin(T.az[k]);
   }
-}
-
-template <int... S>
-__device__ __forceinline__ void sym_phase(std::integer_sequence<int, S...>, SymTargets& T, float xj, float yj,
-                                          float zj, float& bx, float& by, float& bz, float a, float linv) {
-  (sym_step<S>(T, xj, yj, zj, bx, by, bz, a, linv), ...);
 }
 
 __device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& zj, float& bx, float& by,
@@ -430,6 +443,32 @@ __device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& 
   bx = __shfl_xor(bx, mask, 64);
   by = __shfl_xor(by, mask, 64);
   bz = __shfl_xor(bz, mask, 64);
+}
+
+template <int MT, int CUT, int... S>
+__device__ __forceinline__ void sym_phase(std::integer_sequence<int, S...>, SymT<MT>& T, float xj, float yj,
+                                          float zj, float& bx, float& by, float& bz, const SymLaw& L) {
+  (sym_step<S, MT, CUT>(T, xj, yj, zj, bx, by, bz, L), ...);
+}
+
+// The 64 x 64 pairs of the wave's source sub-block (xj, yj, zj, one per lane)
+// with its MT target sub-blocks: four phases of 16 rotation steps, rows
+// exchanging sources between phases.  On return (bx, by, bz) hold the
+// reaction on the lane's own source and T.a* the targets' accumulators.
+template <int MT, int CUT>
+__device__ __forceinline__ void sym_tile(SymT<MT>& T, float xj, float yj, float zj, float& bx, float& by, float& bz,
+                                         const SymLaw& L) {
+  const auto seq = std::make_integer_sequence<int, 16>{};
+  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
+  sym_xchg(16, xj, yj, zj, bx, by, bz);
+  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
+  sym_xchg(32, xj, yj, zj, bx, by, bz);
+  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
+  sym_xchg(16, xj, yj, zj, bx, by, bz);
+  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
+  bx = __shfl_xor(bx, 32, 64);
+  by = __shfl_xor(by, 32, 64);
+  bz = __shfl_xor(bz, 32, 64);
 }
 
 // Work-group (I, k) of the launch-order table wg_tab covers target block I and
@@ -465,23 +504,13 @@ __global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __rest
   }
   const float4* src = pos + (size_t)J0 * SYM_TB + lane;
   float* spw = sp + slot * 3 * sp_len;
-  const auto seq = std::make_integer_sequence<int, 16>{};
+  const SymLaw L{a, linv, 0.f, 0.f, 0.f};
   for (int b = 0; b < nsub; ++b) {
     const float4 p = src[(size_t)b * 64];
-    float xj = p.x, yj = p.y, zj = p.z;
     float bx = 0.f, by = 0.f, bz = 0.f;
 #pragma unroll
     for (int k = 0; k < SYM_MT; ++k) T.ax[k] = T.ay[k] = T.az[k] = 0.f;
-    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
-    sym_xchg(16, xj, yj, zj, bx, by, bz);
-    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
-    sym_xchg(32, xj, yj, zj, bx, by, bz);
-    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
-    sym_xchg(16, xj, yj, zj, bx, by, bz);
-    sym_phase(seq, T, xj, yj, zj, bx, by, bz, a, linv);
-    bx = __shfl_xor(bx, 32, 64);
-    by = __shfl_xor(by, 32, 64);
-    bz = __shfl_xor(bz, 32, 64);
+    sym_tile<SYM_MT, 0>(T, p.x, p.y, p.z, bx, by, bz, L);
 #pragma unroll
     for (int k = 0; k < SYM_MT; ++k) {
       kahan_add(sx[k], cx[k], T.ax[k]);
@@ -1341,6 +1370,209 @@ __global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric cell-list force (cutoff mode, FP32c): every unordered pair within
+// the stencil once.  Work-group = one wave = (target cell c, forward set k):
+// k < 12 are the stencil rows (dy, dz) with dz > 0 or (dz == 0, dy > 0), each
+// 5 cells wide in x (contiguous in cell order); k = 12 is cells c+1, c+2 of
+// c's own row.  The cell's targets are taken 256 at a time (4 sub-blocks of
+// 64 in registers), the set's sources 64 at a time, each 64 x 64 tile by the
+// rotation scheme of the all-pairs kernel with the truncated law.  Partials:
+//   target  tp[k][3][n_pad]  (FP64, sorted position; k = 13: own-cell pairs)
+//   source  sp[slot][3][n_pad] (FP32), slot = 5 k + (source cell x - cx + 2)
+//           for k < 12, 60 + (source cell x - cx - 1) for k = 12,
+// so no two work-groups write the same entry; a work-group adds its later
+// target groups' reactions to its own earlier ones in order.  The pairs
+// inside one cell are done ordered (k_symcell_self), and k_symcell_reduce
+// adds every ion's entries in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int SC_MT = 4;            // target sub-blocks per wave
+constexpr int SC_TG = 64 * SC_MT;   // targets per group
+constexpr int SC_NK = 13;           // forward source sets per cell
+constexpr int SC_NS = 62;           // source partial slots
+
+// forward stencil row k < 12 -> (dy, dz)
+__device__ __forceinline__ void sc_row(int k, int& dy, int& dz) {
+  if (k < 2) {
+    dy = k + 1;
+    dz = 0;
+  } else {
+    dz = 1 + (k - 2) / 5;
+    dy = (k - 2) % 5 - 2;
+  }
+}
+
+// source range [s0, s1) of set k of cell (cx, cy, cz); empty if outside the grid
+__device__ __forceinline__ void sc_sources(const int* __restrict__ cell_start, int nx, int ny, int nz, int cx,
+                                           int cy, int cz, int k, int& s0, int& s1) {
+  s0 = s1 = 0;
+  if (k < 12) {
+    int dy, dz;
+    sc_row(k, dy, dz);
+    const int ry = cy + dy, rz = cz + dz;
+    if (ry < 0 || ry >= ny || rz >= nz) return;
+    const int row = nx * (ry + ny * rz);
+    s0 = cell_start[row + max(cx - 2, 0)];
+    s1 = cell_start[row + min(cx + 2, nx - 1) + 1];
+  } else {
+    if (cx + 1 >= nx) return;
+    const int c = cx + nx * (cy + ny * cz);
+    s0 = cell_start[c + 1];
+    s1 = cell_start[c + min(2, nx - 1 - cx) + 1];
+  }
+}
+
+template <int CUT>
+__global__ __launch_bounds__(64) void k_pair32_symcell(const DevParams* __restrict__ P,
+                                                       const float4* __restrict__ spos,
+                                                       const uint32_t* __restrict__ skey,
+                                                       const int* __restrict__ cell_start,
+                                                       double* __restrict__ tp, float* __restrict__ sp) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz, np = P->n_pad;
+  const int c = blockIdx.x / SC_NK, k = blockIdx.x % SC_NK;
+  const int lane = threadIdx.x;
+  const int t0 = cell_start[c], t1 = cell_start[c + 1];
+  if (t0 == t1) return;
+  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+  int s0, s1;
+  sc_sources(cell_start, nx, ny, nz, cx, cy, cz, k, s0, s1);
+  const SymLaw L{P->a32, P->linv32, P->rc32, P->rs32, P->invw32};
+  const float4 park = make_float4(PARK32, PARK32, PARK32, 0.f);
+  for (int g0 = t0; g0 < t1; g0 += SC_TG) {
+    SymT<SC_MT> T;
+    float sx[SC_MT], sy[SC_MT], sz[SC_MT], ccx[SC_MT], ccy[SC_MT], ccz[SC_MT];
+#pragma unroll
+    for (int m = 0; m < SC_MT; ++m) {
+      const int t = g0 + m * 64 + lane;
+      const float4 q = (t < t1) ? spos[t] : park;
+      T.x[m] = q.x;
+      T.y[m] = q.y;
+      T.z[m] = q.z;
+      sx[m] = sy[m] = sz[m] = ccx[m] = ccy[m] = ccz[m] = 0.f;
+    }
+    for (int b0 = s0; b0 < s1; b0 += 64) {
+      const int j = b0 + lane;
+      const bool jv = j < s1;
+      const float4 q = jv ? spos[j] : park;
+      float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+      for (int m = 0; m < SC_MT; ++m) T.ax[m] = T.ay[m] = T.az[m] = 0.f;
+      sym_tile<SC_MT, CUT>(T, q.x, q.y, q.z, bx, by, bz, L);
+#pragma unroll
+      for (int m = 0; m < SC_MT; ++m) {
+        kahan_add(sx[m], ccx[m], T.ax[m]);
+        kahan_add(sy[m], ccy[m], T.ay[m]);
+        kahan_add(sz[m], ccz[m], T.az[m]);
+      }
+      if (jv) {
+        const int jx = (int)(skey[j] % (uint32_t)nx);
+        const int slot = (k < 12) ? 5 * k + (jx - cx + 2) : 60 + (jx - cx - 1);
+        float* o = sp + (size_t)slot * 3 * np + j;
+        if (g0 == t0) {
+          o[0] = bx;
+          o[np] = by;
+          o[2 * (size_t)np] = bz;
+        } else {
+          o[0] += bx;
+          o[np] += by;
+          o[2 * (size_t)np] += bz;
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < SC_MT; ++m) {
+      const int t = g0 + m * 64 + lane;
+      if (t < t1) {
+        double* o = tp + (size_t)k * 3 * np + t;
+        o[0] = (double)sx[m] - (double)ccx[m];
+        o[np] = (double)sy[m] - (double)ccy[m];
+        o[2 * (size_t)np] = (double)sz[m] - (double)ccz[m];
+      }
+    }
+  }
+}
+
+// pairs inside each cell, ordered (each target against every source of its
+// cell), into target slot 13; sources are work-group-uniform
+template <int SW>
+__global__ __launch_bounds__(256) void k_symcell_self(const DevParams* __restrict__ P, const float4* __restrict__ spos,
+                                                      const int* __restrict__ cell_start, double* __restrict__ tp) {
+  const CutLaw<float, SW> law(*P);
+  const int np = P->n_pad;
+  const int c = blockIdx.x;
+  const int t0 = cell_start[c], t1 = cell_start[c + 1];
+  for (int t = t0 + (int)threadIdx.x; t - (int)threadIdx.x < t1; t += 256) {
+    const bool tv = t < t1;
+    const float4 own = tv ? spos[t] : spos[t0];
+    Acc<float> ax, ay, az;
+    for (int jt = t0; jt < t1; jt += TJ) {
+      const int je = min(jt + TJ, t1);
+      for (int j = jt; j < je; ++j) law(own, spos[j], ax, ay, az);
+      ax.flush();
+      ay.flush();
+      az.flush();
+    }
+    if (tv) {
+      double* o = tp + (size_t)13 * 3 * np + t;
+      o[0] = ax.total();
+      o[np] = ay.total();
+      o[2 * (size_t)np] = az.total();
+    }
+  }
+}
+
+// F[ion] = pair_scale * (its 14 target partials + the source partials of every
+// work-group whose set holds it), fixed order
+__global__ __launch_bounds__(256) void k_symcell_reduce(const DevParams* __restrict__ P,
+                                                        const uint32_t* __restrict__ skey,
+                                                        const uint32_t* __restrict__ sval,
+                                                        const int* __restrict__ cell_start,
+                                                        const double* __restrict__ tp, const float* __restrict__ sp,
+                                                        double* __restrict__ F) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz, np = P->n_pad;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= cell_start[P->n_cells]) return;
+  const int c = (int)skey[p];
+  const int jx = c % nx, jy = (c / nx) % ny, jz = c / (nx * ny);
+  double s[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double v = 0.0;
+    // every work-group (c, k) of a non-empty cell wrote its targets' entries,
+    // zero when its set is empty; slot 13 holds the own-cell pairs
+    for (int k = 0; k < 14; ++k) v += tp[((size_t)k * 3 + q) * np + p];
+    s[q] = v;
+  }
+  // reactions: work-groups (c', k) whose set k holds this ion's cell
+  for (int k = 0; k < SC_NK; ++k) {
+    int dy = 0, dz = 0, dlo, dhi;
+    if (k < 12) {
+      sc_row(k, dy, dz);
+      dlo = -2;
+      dhi = 2;
+    } else {
+      dlo = 1;
+      dhi = 2;
+    }
+    const int cy = jy - dy, cz = jz - dz;
+    if (cy < 0 || cy >= ny || cz < 0 || cz >= nz) continue;
+    for (int d = dlo; d <= dhi; ++d) {
+      const int cx = jx - d;
+      if (cx < 0 || cx >= nx) continue;
+      const int cc = cx + nx * (cy + ny * cz);
+      if (cell_start[cc] == cell_start[cc + 1]) continue;
+      const int slot = (k < 12) ? 5 * k + (d + 2) : 60 + (d - 1);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s[q] += (double)sp[((size_t)slot * 3 + q) * np + p];
+    }
+  }
+  const int i = (int)sval[p];
+  const double sc = P->pair_scale;
+  F[i] = s[0] * sc;
+  F[np + i] = s[1] * sc;
+  F[2 * (size_t)np + i] = s[2] * sc;
+}
+
 // Rows of listed ions in cutoff mode: kept sources from the cell stencil of the
 // ion's cell (FP64 position X) plus the other listed ions `others`.
 template <class T>
@@ -1544,8 +1776,11 @@ struct od_ctx {
   int* wg_base = nullptr;
   double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
   float* sym_sp = nullptr;
-  // cutoff mode: cell list; Gcut: flagged sources' share of the drift-state force
+  // cutoff mode: cell list; Gcut: flagged sources' share of the drift-state force;
+  // sc_tp, sc_sp: partials of the symmetric cell kernel (FP32c)
   double* Gcut = nullptr;
+  double* sc_tp = nullptr;
+  float* sc_sp = nullptr;
   int *scan_tsum = nullptr, *scan_toff = nullptr;
   int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
@@ -1659,7 +1894,7 @@ static void free_ctx(od_ctx* c) {
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
-                  c->rowD, c->sym_sp, c->Gcut, c->scan_tsum, c->scan_toff, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
+                  c->rowD, c->sym_sp, c->Gcut, c->scan_tsum, c->scan_toff, c->sc_tp, c->sc_sp, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
                   c->rs_off, c->rs_total, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
@@ -1785,6 +2020,9 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
     return rc;
   const size_t n_tiles = (std::max<size_t>((size_t)256 * c->rs_nblk, (size_t)c->n_cells + 1) + 1023) / 1024;
   if ((rc = dalloc(&c->scan_tsum, n_tiles * 4)) || (rc = dalloc(&c->scan_toff, n_tiles * 4))) return rc;
+  if (c->prec == OD_PREC_FP32C &&
+      ((rc = dalloc(&c->sc_tp, (size_t)14 * 3 * np * 8)) || (rc = dalloc(&c->sc_sp, (size_t)SC_NS * 3 * np * 4))))
+    return rc;
   return OD_OK;
 }
 
@@ -1982,13 +2220,31 @@ static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const ui
   if (!c->cut) return launch_pair(c, pos, pos, F, timed);
   CHKRET(build_cells_t<T>(c, X, flags, pos));
   const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
+  const bool sw = c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff;
   if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff)
+  if constexpr (std::is_same<T, float>::value) {
+    // symmetric: stencil half-sets, own-cell pairs, per-ion reduction
+    const float4* sp4 = (const float4*)c->spos;
+    if (sw) {
+      hipLaunchKernelGGL((k_pair32_symcell<2>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
+                         c->ckey[0], c->cell_start, c->sc_tp, c->sc_sp);
+      hipLaunchKernelGGL((k_symcell_self<1>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
+                         c->sc_tp);
+    } else {
+      hipLaunchKernelGGL((k_pair32_symcell<1>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
+                         c->ckey[0], c->cell_start, c->sc_tp, c->sc_sp);
+      hipLaunchKernelGGL((k_symcell_self<0>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
+                         c->sc_tp);
+    }
+    hipLaunchKernelGGL(k_symcell_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->ckey[0],
+                       c->cval[0], c->cell_start, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
+  } else if (sw) {
     hipLaunchKernelGGL((k_pair_cell<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
                        c->cval[0], c->cell_start, c->items, c->n_items, F);
-  else
+  } else {
     hipLaunchKernelGGL((k_pair_cell<T, 0>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
                        c->cval[0], c->cell_start, c->items, c->n_items, F);
+  }
   if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
   return OD_OK;
 }
