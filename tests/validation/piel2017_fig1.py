@@ -44,6 +44,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="")
     ap.add_argument("--cases", default="consistent,oml,given")
+    ap.add_argument("--seed", type=int, default=0, help="offset of the Philox key and the NumPy seed")
     args = ap.parse_args()
     from opendust_amd import workloads
 
@@ -53,12 +54,12 @@ def main():
         t0 = time.time()
         print("%s: N %d, 3000 steps" % (case, N), flush=True)
         with contextlib.redirect_stdout(io.StringIO()):
-            od = workloads.piel_fig1(N=N, n=3000, charge=case, q_e=q_e, precision="fp32c", seed=101 + k,
-                                     trapSeed=42)
+            od = workloads.piel_fig1(N=N, n=3000, charge=case, q_e=q_e, precision="fp32c",
+                                     seed=101 + k + 1000 * args.seed, trapSeed=42 + args.seed)
         # progress lines every 500 steps (no files are written)
         od.outputParameters.nOutput = 500
         od.outputParameters.nFileOutput = 500
-        np.random.seed(7 + k)
+        np.random.seed(7 + k + 1000 * args.seed)
         od.simulate("0")
         d = od.dustParticles[0]
         orbit = block_mean(d.forceIonsOrbit[1001:, 2])
