@@ -191,3 +191,44 @@ def test_batch_tune_is_reproducible():
     assert np.array_equal(outs[0][0], outs[1][0])
     for (x, v), (x1, v1) in zip(outs[0][1], outs[1][1]):
         assert np.array_equal(x, x1) and np.array_equal(v, v1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_batch_mixed_geometry_and_distributions_vs_oracle(oracle_lib, precision):
+    """Simulations that differ in everything a configuration holds: cylinder
+    size, Maxwellian vs field-driven flow with collisions, dust count, ion
+    count; each checked against the oracle alone."""
+    import contextlib
+    import io
+
+    from opendust_amd import engine, sweep, workloads
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        ods = [workloads.single_dust(N=2500, n=10, precision=precision, seed=31),
+               workloads.dust_chain(N=3100, n=10, precision=precision, seed=32),
+               workloads.piel_fig1(N=1200, n=10, precision=precision, seed=33, q_e=1e5)]
+    alphas = [workloads.synthetic_alpha(od) for od in ods]
+    np.random.seed(8)
+    cfgs, states = sweep.prepare_batch(ods, device=0, alphas=alphas)
+    assert len({(c.R, c.H) for c in cfgs}) == 3 and len({c.distribution for c in cfgs}) == 2
+    say("batch mixed", precision, [(c.n_ions, c.n_dust, c.distribution) for c in cfgs])
+    b = engine.Batch(cfgs)
+    for s, (x0, v0) in enumerate(states):
+        b.set_ions(s, x0, v0)
+    nsteps = 4 if precision == "fp64" else 1
+    rec = b.run(nsteps)
+    for s, (cfg, (x0, v0)) in enumerate(zip(cfgs, states)):
+        nd = cfg.n_dust
+        x_g, v_g = b.get_ions(s)
+        x_o, v_o, q_o, rec_o = oracle_lib.run(cfg, x0, v0, _q0(cfg), 0, nsteps)
+        r = rec[:, s, :nd]
+        assert np.array_equal(r["absorbed"], rec_o["absorbed"]), s
+        if precision == "fp64":
+            assert np.max(np.abs(x_g - x_o)) <= 1e-10 * cfg.R, s
+            assert np.max(np.abs(r["q"] - rec_o["q"]) / np.abs(rec_o["q"])) <= 1e-12, s
+            sc = np.max(np.abs(rec_o["force_orbit"]))
+            assert np.max(np.abs(r["force_orbit"] - rec_o["force_orbit"])) <= 1e-9 * sc, s
+        else:
+            assert np.max(np.abs(v_g - v_o)) <= 1e-4 * np.max(np.abs(v_o)), s
+            assert np.max(np.abs(r["q"] - rec_o["q"]) / np.abs(rec_o["q"])) <= 1e-12, s
+    b.close()
