@@ -134,8 +134,9 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
     b = engine.Batch(cfgs)
     for s_, (x0, v0) in enumerate(states):
         b.set_ions(s_, x0, v0)
-    chunk = b.tune(3)
-    log("sweep: autotuner chose %d source blocks per work-group" % chunk)
+    chunk, tile = b.tune(3)
+    log("sweep: autotuner chose %d source blocks per work-group, kernel variant %d %s"
+        % (chunk, tile, engine.sym_variants()[tile - 1]))
     b.run(3, engine.RUN_PROFILE_PAIR, records=False)
     b.pair_timing(reset=True)
     b.run(3, engine.RUN_PROFILE_PAIR, records=False)
@@ -171,6 +172,7 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
         "batch_pair_kernel_tflops_alg": ppl * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12,
         "timed_steps": {"batched": steps, "sequential_per_sim": seq_steps},
         "tuned_sym_chunk": chunk,
+        "tuned_sym_tile": tile,
     }
 
 
@@ -193,6 +195,9 @@ def main():
     ap.add_argument("--sym-chunk", type=int, default=0,
                     help="source blocks per work-group of the symmetric kernel (0: autotune, or the default "
                          "with --no-tune)")
+    ap.add_argument("--sym-tile", type=int, default=0,
+                    help="symmetric kernel variant (1-based; tile shape and waves per SIMD); 0: autotune, or "
+                         "variant 1 with --no-tune or --sym-chunk")
     ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
     args = ap.parse_args()
 
@@ -214,15 +219,17 @@ def main():
     cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=7, precision="fp32c")
     cfg.pair_mode = {"rows": engine.PAIR_ROWS, "sym": engine.PAIR_SYM, "auto": engine.PAIR_AUTO}[args.pair_mode]
     cfg.sym_chunk = args.sym_chunk
+    cfg.sym_tile = args.sym_tile
     log("setup done in %.1f s (p_t=%.4f p_s=%.4f)" % (time.time() - t0, od.p_t, od.p_s))
 
     eng = engine.Engine(cfg)
     eng.set_ions(x0, v0)
     mode, pairs_launch = eng.pair_info()
-    chunk = args.sym_chunk
-    if not args.no_tune and not args.pair_only and not args.sym_chunk:
-        chunk = eng.tune(2)
-        log("autotuner chose %d source blocks per work-group" % chunk)
+    chunk, tile = args.sym_chunk, max(args.sym_tile, 1)
+    if not args.no_tune and not args.pair_only and not args.sym_chunk and not args.sym_tile:
+        chunk, tile = eng.tune(2)
+        log("autotuner chose %d source blocks per work-group, kernel variant %d %s"
+            % (chunk, tile, engine.sym_variants()[tile - 1]))
 
     if args.pair_only:
         for _ in range(args.pair_only):
@@ -324,14 +331,18 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            # profiled per schedule (the untuned default is 16 source blocks per work-group at 2^20)
-            "traffic": SYM_TRAFFIC_2E20.get(chunk or 16) if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
+            # profiled per schedule with kernel variant 1 (the untuned default is 16 source blocks per
+            # work-group at 2^20)
+            "traffic": SYM_TRAFFIC_2E20.get(chunk or 16) if (mode == engine.PAIR_SYM and N == 2 ** 20
+                                                              and tile == 1) else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
             "flop_per_pair": FLOP_PER_PAIR,
             "pairs_per_launch": pairs_launch,
             "sym_chunk": chunk,
+            "sym_tile": tile,
+            "sym_tile_shape": dict(zip(("targets_per_lane", "waves_per_simd"), engine.sym_variants()[tile - 1])),
         },
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define OD_ABI_VERSION 1
+#define OD_ABI_VERSION 2
 #define OD_MAX_DUST 8
 #define OD_MAX_TABLE 512
 
@@ -85,6 +85,9 @@ typedef struct od_config {
   int32_t pair_mode;     /* OD_PAIR_*: which all-pairs kernel */
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
+  int32_t sym_tile;      /* symmetric kernel variant (tile shape and waves per SIMD), 1..od_sym_variants();
+                            0 = automatic (variant 1) */
+  int32_t pad0_;
   uint64_t seed;         /* Philox4x32-10 key */
 
   /* geometry, integration (OpenMM units) */
@@ -203,14 +206,24 @@ int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t res
 int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
 /* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
- * return *sym_chunk = 0 and change nothing): times the whole pair evaluation
- * `reps` times for 1, 2, 4, 8, 16 and 32 source blocks per work-group on the
- * current ions, keeps the fastest, and recomputes the ion-ion force of the
- * current positions with it.  The number of blocks groups the partial sums,
- * so results are reproducible for a given choice: *sym_chunk returns it, and
- * od_config.sym_chunk fixes it.  Call after od_set_ions.  No counterpart in the
+ * return *sym_chunk = *sym_tile = 0 and change nothing).  It times the whole
+ * pair evaluation `reps` times per candidate on the current ions: first 1, 2,
+ * 4, 8, 16 and 32 source blocks per work-group with the configured kernel
+ * variant, then every kernel variant (tile shape: 4, 8 or 16 64-target
+ * sub-blocks per wave; register budget: waves per SIMD) at the best of those,
+ * then the neighbouring block counts with the best variant.  It keeps the
+ * fastest and recomputes the ion-ion force of the current positions with it.
+ * Both choices group the partial sums, so results are reproducible for a
+ * given pair: *sym_chunk and *sym_tile return it, and od_config.sym_chunk and
+ * od_config.sym_tile fix it.  Call after od_set_ions.  No counterpart in the
  * reference (OpenMM picks its kernels internally). */
-int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk);
+int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile);
+
+/* Number of symmetric kernel variants, and the tile shape (64-target
+ * sub-blocks per wave) and waves per SIMD of variant v (1-based) as
+ * compiled.  Measurement support. */
+int od_sym_variants(void);
+int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd);
 
 /* Cutoff mode only: the cell list of the last force evaluation.  cell_of_ion[n]
  * (binned ions: their cell; ions left out: the number of cells), sorted_ions
@@ -305,9 +318,10 @@ int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* launches, int64
                          int32_t reset);
 
 /* od_tune for a batch (all simulations share one schedule; the choice is
- * fixed through the first configuration's sym_chunk).  Call after the
- * od_batch_set_ions calls; the next od_batch_run recomputes the forces. */
-int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk);
+ * fixed through the first configuration's sym_chunk and sym_tile).  Call
+ * after the od_batch_set_ions calls; the next od_batch_run recomputes the
+ * forces. */
+int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile);
 
 int od_batch_destroy(od_batch* b);
 

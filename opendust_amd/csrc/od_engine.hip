@@ -159,6 +159,29 @@ __device__ __forceinline__ void pair_acc<double>(const DevParams& P, const doubl
                az.a);
 }
 
+// The untruncated law with its constants read once (registers), for loops
+// that only run in all-pairs mode: no per-pair branch on the cutoff flags.
+template <class T>
+struct PlainLaw;
+template <>
+struct PlainLaw<float> {
+  float a, linv;
+  __device__ explicit PlainLaw(const DevParams& P) : a(P.a32), linv(P.linv32) {}
+  __device__ __forceinline__ void operator()(const float4& i, const float4& j, Acc<float>& ax, Acc<float>& ay,
+                                             Acc<float>& az) const {
+    pair32(i.x, i.y, i.z, j.x, j.y, j.z, a, linv, ax.a, ay.a, az.a);
+  }
+};
+template <>
+struct PlainLaw<double> {
+  double rho0;
+  __device__ explicit PlainLaw(const DevParams& P) : rho0(P.rho0) {}
+  __device__ __forceinline__ void operator()(const double4& i, const double4& j, Acc<double>& ax, Acc<double>& ay,
+                                             Acc<double>& az) const {
+    pair64(i.x, i.y, i.z, j.x, j.y, j.z, rho0, ax.a, ay.a, az.a);
+  }
+};
+
 // parameters of the simulation a 256-ion block belongs to (blk_sim == null: one simulation)
 __device__ __forceinline__ const DevParams* sim_of_block(const DevParams* Pall, const int* blk_sim, int blk) {
   return blk_sim ? Pall + blk_sim[blk] : Pall;
@@ -297,23 +320,22 @@ __global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__
 //
 // Ions are cut into blocks of SYM_TB.  A workgroup owns one target block I
 // and `ch` consecutive source blocks J > I (upper triangle); pairs inside a
-// block are done by k_pair32_diag.  Inside a workgroup (two waves) each wave
-// keeps 8 target sub-blocks of 64 ions in registers and streams 64-ion source
+// block are done by k_pair32_diag.  Inside a workgroup (SYM_TB / (64 MT)
+// waves) each wave keeps MT target sub-blocks of 64 ions in registers (MT = 8
+// by default; 4 and 16 are autotuner variants) and streams 64-ion source
 // sub-blocks.  Within a 16-lane row, step s pairs lane c's targets with the
 // source of lane (c - s) mod 16: the source position is read through a DPP
 // row-rotate folded into the subtraction, and the source's reaction
 // accumulator is rotated by one lane per step, so after 16 steps it is home.
 // Rows exchange sources (xor 16 / xor 32) between the four phases, which
-// covers all 64 x 64 pairs.  The two waves' reactions on a source sub-block
+// covers all 64 x 64 pairs.  The waves' reactions on a source sub-block
 // are summed in a fixed order through LDS and stored as one partial per
 // (workgroup, source); target sums are Kahan-compensated across sub-blocks.
 // k_sym_reduce adds diagonal, target and source partials in a fixed order.
 // 16 + 3 VALU per unordered pair (3 of them transcendental) plus one
-// 3-register rotate per 8 pairs.
+// 3-register rotate per MT pairs.
 // ---------------------------------------------------------------------------
 constexpr int SYM_TB = 1024;  // ions per block of the triangular schedule
-constexpr int SYM_MT = 8;     // 64-target sub-blocks per wave
-constexpr int SYM_NW = SYM_TB / (64 * SYM_MT);  // waves per workgroup
 
 template <int S>
 __device__ __forceinline__ float dpp_ror(float v) {
@@ -331,7 +353,6 @@ struct SymT {
   float x[MT], y[MT], z[MT];
   float ax[MT], ay[MT], az[MT];
 };
-using SymTargets = SymT<SYM_MT>;
 
 // pair-law parameters of the symmetric kernels (FP32, units r_D_e / log2 e)
 struct SymLaw {
@@ -477,12 +498,18 @@ __device__ __forceinline__ void sym_tile(SymT<MT>& T, float xj, float yj, float 
 // tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials.
 // Batched simulations: blk_end[I] (first block past I's simulation) and
 // blk_a[I] (its a) per block; null for one simulation (n_blocks, a).
-__global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __restrict__ pos, int n_blocks, int ch,
-                                                    const int2* __restrict__ wg_tab,
-                                                    const int* __restrict__ wg_base, float a, float linv,
-                                                    const int* __restrict__ blk_end, const float* __restrict__ blk_a,
-                                                    double* __restrict__ tp, float* __restrict__ sp) {
-  __shared__ double red[SYM_NW][3][64];
+// Tile shape MT (64-target sub-blocks per wave; the work-group has
+// SYM_TB / (64 MT) waves) and WPE (waves per SIMD the register allocation
+// must allow) are template parameters; the autotuner picks among the
+// instantiated variants (sym_variants).  The next source sub-block's
+// positions are loaded while the current one is processed.
+template <int MT, int WPE>
+__global__ __launch_bounds__(SYM_TB / MT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pair32_sym(
+    const float4* __restrict__ pos, int n_blocks, int ch, const int2* __restrict__ wg_tab,
+    const int* __restrict__ wg_base, float a, float linv, const int* __restrict__ blk_end,
+    const float* __restrict__ blk_a, double* __restrict__ tp, float* __restrict__ sp) {
+  constexpr int NW = SYM_TB / (64 * MT);
+  __shared__ double red[NW][3][64];
   const int2 ik = wg_tab[blockIdx.x];
   const int I = ik.x;
   const size_t slot = (size_t)wg_base[I] + ik.y;
@@ -492,11 +519,11 @@ __global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __rest
   const int nsub = (J1 - J0) * (SYM_TB / 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t sp_len = (size_t)ch * SYM_TB;
-  SymTargets T;
-  float sx[SYM_MT], sy[SYM_MT], sz[SYM_MT], cx[SYM_MT], cy[SYM_MT], cz[SYM_MT];
+  SymT<MT> T;
+  float sx[MT], sy[MT], sz[MT], cx[MT], cy[MT], cz[MT];
 #pragma unroll
-  for (int k = 0; k < SYM_MT; ++k) {
-    const float4 p = pos[(size_t)I * SYM_TB + wave * (64 * SYM_MT) + k * 64 + lane];
+  for (int k = 0; k < MT; ++k) {
+    const float4 p = pos[(size_t)I * SYM_TB + wave * (64 * MT) + k * 64 + lane];
     T.x[k] = p.x;
     T.y[k] = p.y;
     T.z[k] = p.z;
@@ -505,14 +532,15 @@ __global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __rest
   const float4* src = pos + (size_t)J0 * SYM_TB + lane;
   float* spw = sp + slot * 3 * sp_len;
   const SymLaw L{a, linv, 0.f, 0.f, 0.f};
+  float4 p = src[0];
   for (int b = 0; b < nsub; ++b) {
-    const float4 p = src[(size_t)b * 64];
+    const float4 pn = src[(size_t)min(b + 1, nsub - 1) * 64];
     float bx = 0.f, by = 0.f, bz = 0.f;
 #pragma unroll
-    for (int k = 0; k < SYM_MT; ++k) T.ax[k] = T.ay[k] = T.az[k] = 0.f;
-    sym_tile<SYM_MT, 0>(T, p.x, p.y, p.z, bx, by, bz, L);
+    for (int k = 0; k < MT; ++k) T.ax[k] = T.ay[k] = T.az[k] = 0.f;
+    sym_tile<MT, 0>(T, p.x, p.y, p.z, bx, by, bz, L);
 #pragma unroll
-    for (int k = 0; k < SYM_MT; ++k) {
+    for (int k = 0; k < MT; ++k) {
       kahan_add(sx[k], cx[k], T.ax[k]);
       kahan_add(sy[k], cy[k], T.ay[k]);
       kahan_add(sz[k], cz[k], T.az[k]);
@@ -521,24 +549,42 @@ __global__ __launch_bounds__(64 * SYM_NW) void k_pair32_sym(const float4* __rest
     red[wave][1][lane] = by;
     red[wave][2][lane] = bz;
     __syncthreads();
-    for (int e = threadIdx.x; e < 192; e += 64 * SYM_NW) {
+    for (int e = threadIdx.x; e < 192; e += 64 * NW) {
       const int c = e >> 6, l = e & 63;
       double v = red[0][c][l];
 #pragma unroll
-      for (int w = 1; w < SYM_NW; ++w) v += red[w][c][l];
+      for (int w = 1; w < NW; ++w) v += red[w][c][l];
       spw[c * sp_len + (size_t)b * 64 + l] = (float)v;
     }
     __syncthreads();
+    p = pn;
   }
   double* tpw = tp + slot * 3 * SYM_TB;
 #pragma unroll
-  for (int k = 0; k < SYM_MT; ++k) {
-    const int t = wave * (64 * SYM_MT) + k * 64 + lane;
+  for (int k = 0; k < MT; ++k) {
+    const int t = wave * (64 * MT) + k * 64 + lane;
     tpw[t] = (double)sx[k] - (double)cx[k];
     tpw[SYM_TB + t] = (double)sy[k] - (double)cy[k];
     tpw[2 * SYM_TB + t] = (double)sz[k] - (double)cz[k];
   }
 }
+
+using SymKernel = void (*)(const float4*, int, int, const int2*, const int*, float, float, const int*, const float*,
+                           double*, float*);
+// Variants of the symmetric kernel the autotuner chooses from (od_config.sym_tile
+// - 1): tile shape MT and the waves per SIMD the register budget must allow.
+// As compiled for gfx950: <8,3> 145 VGPRs (3 waves per SIMD), <8,4> 128 (4, a
+// few registers spilled), <4,5> 84 (5), <16,2> 256 (2, a few spilled), <16,1>
+// 264 (1).
+struct SymVariant {
+  SymKernel fn;
+  int mt, wpe;
+};
+static const SymVariant sym_variants[] = {
+    {k_pair32_sym<8, 3>, 8, 3},   {k_pair32_sym<8, 4>, 8, 4},   {k_pair32_sym<4, 5>, 4, 5},
+    {k_pair32_sym<16, 2>, 16, 2}, {k_pair32_sym<16, 1>, 16, 1},
+};
+constexpr int SYM_N_VARIANTS = (int)(sizeof(sym_variants) / sizeof(sym_variants[0]));
 
 // pairs inside one block (ordered, sources via scalar loads); dg: [3][n_pad]
 __global__ __launch_bounds__(256) void k_pair32_diag(const float4* __restrict__ pos, int n_pad, float a,
@@ -581,17 +627,28 @@ __global__ __launch_bounds__(256) void k_sym_reduce(const double* __restrict__ d
   if (blk_scale) scale = blk_scale[X];
   const size_t sp_len = (size_t)ch * SYM_TB;
   const int k0 = wg_base[X], k1 = wg_base[X + 1];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double s = dg[(size_t)c * n_pad + i];
-    for (int J = Jb; J < X; ++J) {
-      const int d = X - J - 1;
-      const size_t wg = (size_t)wg_base[J] + d / ch;
-      s += (double)sp[(wg * 3 + c) * sp_len + (size_t)(d % ch) * SYM_TB + off];
-    }
-    for (int w = k0; w < k1; ++w) s += tp[((size_t)w * 3 + c) * SYM_TB + off];
-    F[(size_t)c * n_pad + i] = s * scale;
+  // the three components walk the partials together (three independent
+  // chains of loads in flight); each component's order is J ascending, then w
+  double s0 = dg[i], s1 = dg[(size_t)n_pad + i], s2 = dg[2 * (size_t)n_pad + i];
+#pragma unroll 4
+  for (int J = Jb; J < X; ++J) {
+    const int d = X - J - 1;
+    const size_t wg = (size_t)wg_base[J] + d / ch;
+    const float* q = sp + wg * 3 * sp_len + (size_t)(d % ch) * SYM_TB + off;
+    s0 += (double)q[0];
+    s1 += (double)q[sp_len];
+    s2 += (double)q[2 * sp_len];
   }
+#pragma unroll 4
+  for (int w = k0; w < k1; ++w) {
+    const double* t = tp + (size_t)w * 3 * SYM_TB + off;
+    s0 += t[0];
+    s1 += t[SYM_TB];
+    s2 += t[2 * SYM_TB];
+  }
+  F[i] = s0 * scale;
+  F[(size_t)n_pad + i] = s1 * scale;
+  F[2 * (size_t)n_pad + i] = s2 * scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -844,9 +901,10 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pal
   // flagged sources' share was computed in cell order (cutoff mode)
   const int l0 = block_off[P->blk0];
   const int l1 = G ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
+  const PlainLaw<T> law(*P);  // the list loop runs in all-pairs mode only
   for (int s0 = l0; s0 < l1; s0 += TJ) {
     const int s1 = min(s0 + TJ, l1);
-    for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_old[s], ax, ay, az);
+    for (int s = s0; s < s1; ++s) law(own, list_old[s], ax, ay, az);
     ax.flush();
     ay.flush();
     az.flush();
@@ -927,35 +985,103 @@ __global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pal
 // Full force rows of listed ions against all sources `pj`: the reinjected ions
 // at the post-reinjection state (need_mask = 0), or the absorbed ions at the
 // drifted state (need_mask = dust bits: other listed ions are skipped).
+// A work-group of RowShape<T>::TB threads takes RowShape<T>::G consecutive
+// list slots and streams its simulation's sources once for all of them (each
+// source load feeds G pairs); slots of another simulation in the same group
+// (batched runs) get a pass of their own.  FP32: 4 slots, 16 waves (about
+// 1200 reinjected ions per step at 2^20 give ~300 work-groups, each 16 waves
+// deep for latency hiding); FP64: 2 slots, 8 waves (twice the groups for the
+// heavier pairs).  Per thread: sources strided by TB, Kahan flush every
+// TJ of them; then a fixed-order sum over the work-group.  All-pairs mode only
+// (the cell-list mode has k_rows_cell).
 template <class T>
-__global__ __launch_bounds__(256) void k_rows(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
-                                              const typename Pos<T>::V* __restrict__ pj, const int* __restrict__ n_flag,
-                                              const typename Pos<T>::V* __restrict__ list_new,
-                                              const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
-                                              uint32_t need_mask, double* __restrict__ rowF) {
-  __shared__ double lds[4];
+struct RowShape;
+template <>
+struct RowShape<float> {
+  static constexpr int G = 4, TB = 1024;
+};
+template <>
+struct RowShape<double> {
+  static constexpr int G = 2, TB = 512;
+};
+
+template <class T>
+__global__ __launch_bounds__(RowShape<T>::TB) void k_rows(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
+                                                 const typename Pos<T>::V* __restrict__ pj,
+                                                 const int* __restrict__ n_flag,
+                                                 const typename Pos<T>::V* __restrict__ list_new,
+                                                 const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
+                                                 uint32_t need_mask, double* __restrict__ rowF) {
+  using PV = typename Pos<T>::V;
+  constexpr int ROW_G = RowShape<T>::G, ROW_TB = RowShape<T>::TB;
+  constexpr int NW = ROW_TB / 64, NV = 3 * ROW_G;
+  __shared__ double lds[NW][NV];
   const int nf = n_flag[0];
-  for (int s = blockIdx.x; s < nf; s += gridDim.x) {
-    const int ion = list_ion[s];
-    if (need_mask != 0u && (flags[ion] & need_mask) == 0u) continue;
-    const DevParams* P = sim_of_block(Pall, blk_sim, ion >> 8);
-    const typename Pos<T>::V own = list_new[s];
-    Acc<T> ax, ay, az;
-    const int jb = P->ion_off, je = P->ion_off + P->n_pad;
-    for (int j0 = jb + (int)threadIdx.x; j0 < je; j0 += 256 * TJ) {
-      const int j1 = min(j0 + 256 * TJ, je);
-      for (int j = j0; j < j1; j += 256) pair_acc<T>(*P, own, pj[j], ax, ay, az);
-      ax.flush();
-      ay.flush();
-      az.flush();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int s0 = blockIdx.x * ROW_G; s0 < nf; s0 += gridDim.x * ROW_G) {
+    uint32_t todo = 0;
+#pragma unroll
+    for (int k = 0; k < ROW_G; ++k) {
+      const int s = s0 + k;
+      if (s < nf && (need_mask == 0u || (flags[list_ion[s]] & need_mask) != 0u)) todo |= 1u << k;
     }
-    const double fx = block_sum256(ax.total(), lds);
-    const double fy = block_sum256(ay.total(), lds);
-    const double fz = block_sum256(az.total(), lds);
-    if (threadIdx.x == 0) {
-      rowF[3 * s + 0] = fx * P->pair_scale;
-      rowF[3 * s + 1] = fy * P->pair_scale;
-      rowF[3 * s + 2] = fz * P->pair_scale;
+    while (todo) {
+      const int sim_blk = list_ion[s0 + __builtin_ctz(todo)] >> 8;
+      const DevParams* P = sim_of_block(Pall, blk_sim, sim_blk);
+      uint32_t mine = 0;
+      PV own[ROW_G];
+#pragma unroll
+      for (int k = 0; k < ROW_G; ++k) {
+        own[k] = Pos<T>::park();
+        if (((todo >> k) & 1u) && sim_of_block(Pall, blk_sim, list_ion[s0 + k] >> 8) == P) {
+          mine |= 1u << k;
+          own[k] = list_new[s0 + k];
+        }
+      }
+      todo &= ~mine;
+      Acc<T> ax[ROW_G], ay[ROW_G], az[ROW_G];
+      const PlainLaw<T> law(*P);
+      const int jb = P->ion_off, je = P->ion_off + P->n_pad;
+      for (int j0 = jb + (int)threadIdx.x; j0 < je; j0 += ROW_TB * TJ) {
+        const int j1 = min(j0 + ROW_TB * TJ, je);
+#pragma unroll 4
+        for (int j = j0; j < j1; j += ROW_TB) {
+          const PV p = pj[j];
+#pragma unroll
+          for (int k = 0; k < ROW_G; ++k) law(own[k], p, ax[k], ay[k], az[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < ROW_G; ++k) {
+          ax[k].flush();
+          ay[k].flush();
+          az[k].flush();
+        }
+      }
+      double v[NV];
+#pragma unroll
+      for (int k = 0; k < ROW_G; ++k) {
+        v[3 * k + 0] = ax[k].total();
+        v[3 * k + 1] = ay[k].total();
+        v[3 * k + 2] = az[k].total();
+      }
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[m] += __shfl_xor(v[m], o, 64);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) lds[wave][m] = v[m];
+      }
+      __syncthreads();
+      if (threadIdx.x < NV) {
+        const int m = threadIdx.x, k = m / 3;
+        double s = lds[0][m];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += lds[w][m];
+        if ((mine >> k) & 1u) rowF[3 * (s0 + k) + m % 3] = s * P->pair_scale;
+      }
+      __syncthreads();
     }
   }
 }
@@ -978,9 +1104,10 @@ __global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ Pa
   // rows_only: the kept ions' share was added in cell order (cutoff mode)
   const int l0 = block_off[P->blk0];
   const int l1 = rows_only ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
+  const PlainLaw<T> law(*P);  // the list loop runs in all-pairs mode only
   for (int s0 = l0; s0 < l1; s0 += TJ) {
     const int s1 = min(s0 + TJ, l1);
-    for (int s = s0; s < s1; ++s) pair_acc<T>(*P, own, list_new[s], ax, ay, az);
+    for (int s = s0; s < s1; ++s) law(own, list_new[s], ax, ay, az);
     ax.flush();
     ay.flush();
     az.flush();
@@ -1771,7 +1898,7 @@ struct od_ctx {
   double pair_ms = 0.0;
   int64_t pair_launches = 0;
   // symmetric pair kernel
-  int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0;
+  int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0, sym_var = 0;
   int2* wg_tab = nullptr;
   int* wg_base = nullptr;
   double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
@@ -2037,6 +2164,7 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   if (!forced && sym_bytes(be, ch) > 48e9) return OD_OK;
   int rc = OD_OK;
   if ((rc = sym_build(c, be, ch)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8))) return rc;
+  c->sym_var = cfg->sym_tile > 0 ? cfg->sym_tile - 1 : 0;
   c->sym = 1;
   return OD_OK;
 }
@@ -2056,6 +2184,9 @@ static int check_config(const od_config* c) {
     return set_err(OD_ERR_ARG, "unknown precision %d", c->precision);
   if (c->distribution != OD_DIST_MAXWELLIAN && c->distribution != OD_DIST_FIELD_DRIVEN)
     return set_err(OD_ERR_ARG, "unknown distribution %d", c->distribution);
+  if (c->sym_tile < 0 || c->sym_tile > SYM_N_VARIANTS)
+    return set_err(OD_ERR_ARG, "sym_tile %d outside [0, %d]", c->sym_tile, SYM_N_VARIANTS);
+  if (c->sym_chunk < 0) return set_err(OD_ERR_ARG, "sym_chunk %d is negative", c->sym_chunk);
   const od_table* ts[4] = {&c->top, &c->bottom, &c->side, &c->vz};
   for (const od_table* t : ts)
     if (t->n_seg < 0 || t->n_seg > OD_MAX_TABLE) return set_err(OD_ERR_ARG, "table length %d invalid", t->n_seg);
@@ -2141,7 +2272,8 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
   if (c->sym && pi == pj) {
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
     if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_pair32_sym, dim3(c->n_wg), dim3(64 * SYM_NW), 0, c->stream, (const float4*)pj, c->n_blocks,
+    const SymVariant& V = sym_variants[c->sym_var];
+    hipLaunchKernelGGL(V.fn, dim3(c->n_wg), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
                        c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, a, linv, (const int*)nullptr,
                        (const float*)nullptr, c->sym_tp, c->sym_sp);
     if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -2285,7 +2417,7 @@ static int launch_step_t(od_ctx* c, bool timed) {
                          (const PV*)c->spos, c->cell_start, c->n_flag, (const PV*)c->list_old,
                          (const PV*)c->list_old, c->flags, c->list_ion, dust_bits, c->rowD);
     else
-      hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const int*)nullptr,
+      hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(RowShape<T>::TB), 0, c->stream, c->dP, (const int*)nullptr,
                          (const PV*)c->pi, c->n_flag, (const PV*)c->list_old, c->flags, c->list_ion, dust_bits,
                          c->rowD);
   }
@@ -2300,7 +2432,7 @@ static int launch_step_t(od_ctx* c, bool timed) {
                        c->cell_start, c->n_flag, (const PV*)c->list_new, (const PV*)c->list_new, c->flags,
                        c->list_ion, 0u, c->rowF);
   else
-    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, (const int*)nullptr,
+    hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(RowShape<T>::TB), 0, c->stream, c->dP, (const int*)nullptr,
                        (const PV*)c->pj, c->n_flag, (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
   if (c->cut) launch_corr_cell<T>(c, (const PV*)c->list_new, 1, c->F);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
@@ -2392,9 +2524,10 @@ extern "C" int od_update_config(od_ctx* c, const od_config* cfg) {
   CHKRET(fill_params(cfg, c->n_pad, &hp));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(c->dP, &hp, sizeof(hp), hipMemcpyHostToDevice));
-  const int keep_chunk = c->cfg.sym_chunk;
+  const int keep_chunk = c->cfg.sym_chunk, keep_tile = c->cfg.sym_tile;
   c->cfg = *cfg;
   c->cfg.sym_chunk = keep_chunk;
+  c->cfg.sym_tile = keep_tile;
   // captured steps hold kernel arguments derived from the old parameters
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
@@ -2739,12 +2872,13 @@ __global__ __launch_bounds__(256) void k_pair_items(const DevParams* __restrict_
   const typename Pos<T>::V a = pj[it.tgt + threadIdx.x];
   const typename Pos<T>::V b = pj[it.tgt + 256 + threadIdx.x];
   Acc<T> ax0, ay0, az0, ax1, ay1, az1;
+  const PlainLaw<T> law(*P);  // batched sweeps run all-pairs forces
   for (int jt = it.src0; jt < it.src1; jt += TJ) {
 #pragma unroll 8
     for (int j = jt; j < jt + TJ; ++j) {
       const typename Pos<T>::V p = pj[j];
-      pair_acc<T>(*P, a, p, ax0, ay0, az0);
-      pair_acc<T>(*P, b, p, ax1, ay1, az1);
+      law(a, p, ax0, ay0, az0);
+      law(b, p, ax1, ay1, az1);
     }
     ax0.flush();
     ay0.flush();
@@ -2824,7 +2958,7 @@ struct od_batch {
   double pair_ms = 0.0;
   int64_t pair_launches = 0, pairs_per_launch = 0;
   // symmetric FP32 pair kernel over all simulations (1024-ion blocks)
-  int sym = 0, sym_ch = 1, n_wg = 0, n_blocks = 0;
+  int sym = 0, sym_ch = 1, n_wg = 0, n_blocks = 0, sym_var = 0;
   std::vector<int> blk_end_host;
   int2* wg_tab = nullptr;
   int *wg_base = nullptr, *blk_begin = nullptr, *blk_end = nullptr;
@@ -2912,6 +3046,7 @@ extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch**
   }
   b->n_items = (int)items.size();
   b->sym = (b->prec == OD_PREC_FP32C && cfgs[0].pair_mode != OD_PAIR_ROWS) ? 1 : 0;
+  b->sym_var = cfgs[0].sym_tile > 0 ? cfgs[0].sym_tile - 1 : 0;
   const size_t np = b->n_tot;
   int rc = OD_OK;
   if (hipSetDevice(b->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice failed"));
@@ -3044,7 +3179,8 @@ static void batch_pair(od_batch* b, bool timed, double* Fout) {
     const float linv = (float)(1.0 / 1.4426950408889634);
     if (timed) hipEventRecord(b->ev0, b->stream);
     if (b->n_wg > 0)
-      hipLaunchKernelGGL(k_pair32_sym, dim3(b->n_wg), dim3(64 * SYM_NW), 0, b->stream, (const float4*)b->pj, b->n_blocks,
+      hipLaunchKernelGGL(sym_variants[b->sym_var].fn, dim3(b->n_wg), dim3(SYM_TB / sym_variants[b->sym_var].mt), 0,
+                         b->stream, (const float4*)b->pj, b->n_blocks,
                          b->sym_ch, (const int2*)b->wg_tab, (const int*)b->wg_base, 0.0f, linv,
                          (const int*)b->blk_end, (const float*)b->blk_a, b->sym_tp, b->sym_sp);
     if (timed) hipEventRecord(b->ev1, b->stream);
@@ -3073,14 +3209,14 @@ static int batch_step_t(od_batch* b, bool timed) {
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, b->stream, b->flags, b->block_cnt, b->block_off, b->n_tot,
                      (const PV*)b->pi, b->slot_of, b->list_ion, (PV*)b->list_old);
   batch_pair<T>(b, timed, b->F);
-  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pi,
+  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(RowShape<T>::TB), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pi,
                      b->n_flag, (const PV*)b->list_old, b->flags, b->list_ion, (uint32_t)((1u << OD_MAX_DUST) - 1u),
                      b->rowD);
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->X, b->V,
                      b->F, b->q, b->st, (const PV*)b->pi, (PV*)b->pj, b->flags, b->n_flag, b->slot_of,
                      (const PV*)b->list_old, (PV*)b->list_new, b->face, b->rowD, (const double*)nullptr,
                      b->partials);
-  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(256), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pj,
+  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(RowShape<T>::TB), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pj,
                      b->n_flag, (const PV*)b->list_new, b->flags, b->list_ion, 0u, b->rowF);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->F,
                      (const PV*)b->pj, b->flags, b->n_flag, (const PV*)b->list_new, b->slot_of, b->rowF, 0);
@@ -3193,20 +3329,23 @@ extern "C" int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* laun
   return OD_OK;
 }
 
-// Times the symmetric pair evaluation (kernel, diagonal, reduction) for each
-// candidate number of source blocks per work-group on the current positions
-// and keeps the fastest.  Results depend on that number (it groups the partial
-// sums), so runs are reproducible for a given choice, which is returned and
-// can be fixed through od_config.sym_chunk.
+// Autotuner of the symmetric pair evaluation (kernel, diagonal, reduction) on
+// the current positions.  Stage 1 times each candidate number of source blocks
+// per work-group with the configured kernel variant; stage 2 every variant
+// (tile shape, waves per SIMD) at the best block count; stage 3 the
+// neighbouring block counts with the best variant.  Both choices group the
+// partial sums, so runs are reproducible for a given pair, which is returned
+// and can be fixed through od_config.sym_chunk and od_config.sym_tile.
 static const int kTuneChunks[] = {1, 2, 4, 8, 16, 32};
 
 template <class C, class F>
-static int tune_chunks(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen) {
+static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen_ch, int* chosen_var) {
   double best_ms = 1e300;
-  int best = c->sym_ch;
-  for (int ch : kTuneChunks) {
-    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > 48e9) continue;
-    CHKRET(sym_build(c, be, ch));
+  int best_ch = c->sym_ch, best_var = c->sym_var;
+  auto time_one = [&](int ch, int var) -> int {
+    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > 48e9) return OD_OK;
+    if (c->sym_ch != ch) CHKRET(sym_build(c, be, ch));
+    c->sym_var = var;
     evaluate();  // warm-up
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < reps; ++r) evaluate();
@@ -3217,30 +3356,52 @@ static int tune_chunks(C* c, const std::vector<int>& be, int reps, F&& evaluate,
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (ms < best_ms) {
       best_ms = ms;
-      best = ch;
+      best_ch = ch;
+      best_var = var;
     }
+    return OD_OK;
+  };
+  const int var0 = c->sym_var;
+  for (int ch : kTuneChunks) CHKRET(time_one(ch, var0));
+  const int ch1 = best_ch;
+  for (int v = 0; v < SYM_N_VARIANTS; ++v)
+    if (v != var0) CHKRET(time_one(ch1, v));
+  if (best_var != var0) {
+    for (int ch : kTuneChunks)
+      if (ch != ch1 && (ch == 2 * ch1 || 2 * ch == ch1)) CHKRET(time_one(ch, best_var));
   }
-  if (c->sym_ch != best) CHKRET(sym_build(c, be, best));
+  if (c->sym_ch != best_ch) CHKRET(sym_build(c, be, best_ch));
+  c->sym_var = best_var;
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
   c->graph_steps = 0;
-  *chosen = best;
+  *chosen_ch = best_ch;
+  *chosen_var = best_var;
   return OD_OK;
 }
 
-extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk) {
+extern "C" int od_sym_variants(void) { return SYM_N_VARIANTS; }
+
+extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd) {
+  if (v < 1 || v > SYM_N_VARIANTS) return set_err(OD_ERR_ARG, "variant %d outside [1, %d]", v, SYM_N_VARIANTS);
+  if (targets_per_lane) *targets_per_lane = sym_variants[v - 1].mt;
+  if (waves_per_simd) *waves_per_simd = sym_variants[v - 1].wpe;
+  return OD_OK;
+}
+
+extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   if (reps < 1 || reps > 100) return set_err(OD_ERR_ARG, "reps %d outside [1, 100]", reps);
-  int chosen = 0;
+  int chosen = 0, var = -1;
   if (c->sym) {
     if (!c->have_ions) return set_err(OD_ERR_STATE, "od_tune needs the ions (call od_set_ions first)");
     HIPCHK(hipSetDevice(c->dev));
     double* Fs = nullptr;
     CHKRET(dalloc(&Fs, (size_t)3 * c->n_pad * 8));
     const std::vector<int> be(c->n_blocks, c->n_blocks);
-    const int rc = tune_chunks(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen);
+    const int rc = tune_sym(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen, &var);
     hipFree(Fs);
     if (rc != OD_OK) return rc;
     // the ion-ion force of the current positions, with the chosen schedule
@@ -3250,13 +3411,14 @@ extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk) {
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (sym_chunk) *sym_chunk = chosen;
+  if (sym_tile) *sym_tile = var + 1;
   return OD_OK;
 }
 
-extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk) {
+extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile) {
   if (!b) return set_err(OD_ERR_ARG, "null batch");
   if (reps < 1 || reps > 100) return set_err(OD_ERR_ARG, "reps %d outside [1, 100]", reps);
-  int chosen = 0;
+  int chosen = 0, var = -1;
   if (b->sym) {
     HIPCHK(hipSetDevice(b->dev));
     if (b->stale)
@@ -3264,12 +3426,13 @@ extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk) {
                          (float4*)b->pi, (float4*)b->pj);
     double* Fs = nullptr;
     CHKRET(dalloc(&Fs, (size_t)3 * b->n_tot * 8));
-    const int rc = tune_chunks(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen);
+    const int rc = tune_sym(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen, &var);
     hipFree(Fs);
     if (rc != OD_OK) return rc;
     b->stale = true;  // the next run recomputes the forces with the chosen schedule
   }
   if (sym_chunk) *sym_chunk = chosen;
+  if (sym_tile) *sym_tile = var + 1;
   return OD_OK;
 }
 

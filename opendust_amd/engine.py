@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libopendust_amd.so")
 
 MAX_DUST = host.MAX_DUST
 MAX_TABLE = host.MAX_TABLE
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 RUN_GRAPH = 1
 RUN_PROFILE_PAIR = 2
@@ -70,6 +70,8 @@ class Config(ctypes.Structure):
         ("pair_mode", ctypes.c_int32),
         ("sym_chunk", ctypes.c_int32),
         ("sim_id", ctypes.c_int32),
+        ("sym_tile", ctypes.c_int32),
+        ("pad0_", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("R", ctypes.c_double), ("H", ctypes.c_double), ("dt", ctypes.c_double),
         ("r_D_e", ctypes.c_double), ("r_0", ctypes.c_double),
@@ -142,7 +144,7 @@ EXPORTS = [
     "od_trap_potential_grid", "od_trap_samples_host", "od_abi_sizes", "od_destroy",
     "od_last_error", "od_build_info",
     "od_batch_create", "od_batch_set_ions", "od_batch_get_ions", "od_batch_run", "od_batch_get_step_info",
-    "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune",
+    "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune", "od_sym_variants", "od_sym_variant_info",
     "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
     "od_update_config",
 ]
@@ -187,7 +189,9 @@ def load_library(path=LIB_PATH):
     lib.od_batch_get_step_info.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_uint32), P(ctypes.c_uint8)]
     lib.od_batch_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
     lib.od_batch_destroy.argtypes = [c_ctx]
-    lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32)]
+    lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
+    lib.od_sym_variants.argtypes = []
+    lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_update_config.argtypes = [c_ctx, P(Config)]
     lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
     lib.od_write_csv.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
@@ -196,7 +200,7 @@ def load_library(path=LIB_PATH):
                                        dp, dp]
     lib.od_density_profile_z.argtypes = [ctypes.c_int32, dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, dp, dp]
-    lib.od_batch_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32)]
+    lib.od_batch_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_last_error.restype = ctypes.c_char_p
     lib.od_build_info.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -223,6 +227,18 @@ def check_abi():
     rs = ctypes.c_int64()
     _check(lib.od_abi_sizes(ctypes.byref(cs), ctypes.byref(rs)), "od_abi_sizes")
     return cs.value == ctypes.sizeof(Config) and rs.value == ctypes.sizeof(DustRecord)
+
+
+def sym_variants():
+    """[(64-target sub-blocks per wave, waves per SIMD)] of the symmetric kernel
+    variants; od_config.sym_tile = index + 1 selects one."""
+    lib = load_library()
+    out = []
+    for v in range(1, lib.od_sym_variants() + 1):
+        mt, wps = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.od_sym_variant_info(v, ctypes.byref(mt), ctypes.byref(wps)), "od_sym_variant_info")
+        out.append((mt.value, wps.value))
+    return out
 
 
 def trap_potential_grid(R_over_lambda, H_over_lambda, rho, h, n_mc, seed, device=0):
@@ -394,11 +410,14 @@ class Engine:
         return zg, c
 
     def tune(self, reps=3):
-        """Autotune the symmetric kernel's source blocks per work-group (0 when
-        the context does not run it); returns the choice."""
+        """Autotune the symmetric kernel: source blocks per work-group and kernel
+        variant (tile shape, waves per SIMD).  Returns (sym_chunk, sym_tile),
+        (0, 0) when the context does not run it; setting both in the
+        configuration reproduces the tuned run."""
         ch = ctypes.c_int32()
-        _check(self.lib.od_tune(self._ctx, int(reps), ctypes.byref(ch)), "od_tune")
-        return ch.value
+        tile = ctypes.c_int32()
+        _check(self.lib.od_tune(self._ctx, int(reps), ctypes.byref(ch), ctypes.byref(tile)), "od_tune")
+        return ch.value, tile.value
 
     def pair_forces(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
@@ -465,10 +484,12 @@ class Batch:
         return flags, face
 
     def tune(self, reps=3):
-        """Autotune the batch's symmetric kernel (see Engine.tune)."""
+        """Autotune the batch's symmetric kernel (see Engine.tune); returns
+        (sym_chunk, sym_tile), fixed through the first configuration."""
         ch = ctypes.c_int32()
-        _check(self.lib.od_batch_tune(self._b, int(reps), ctypes.byref(ch)), "od_batch_tune")
-        return ch.value
+        tile = ctypes.c_int32()
+        _check(self.lib.od_batch_tune(self._b, int(reps), ctypes.byref(ch), ctypes.byref(tile)), "od_batch_tune")
+        return ch.value, tile.value
 
     def pair_timing(self, reset=False):
         ms = ctypes.c_double()

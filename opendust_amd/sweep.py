@@ -73,15 +73,16 @@ def prepare_batch(openDusts, device=0, alphas=None, precision=None):
 
 
 def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precision=None, tune=False,
-                  sym_chunk=None, progress=None):
+                  sym_chunk=None, sym_tile=None, progress=None):
     """Run every OpenDust object of `openDusts` for its own n steps, together.
     Simulations may differ in ion count, dust count and positions, plasma
     parameters and geometry; they share precision and device.  The batch steps
     until the longest simulation ends; the extra steps of shorter ones are
     computed and discarded.  tune=True lets the autotuner pick the pair
-    kernel's tile shape (od_batch_tune); the choice is returned, and results
-    are reproducible for it (pass it back as sym_chunk).  progress(done, total)
-    is called after every chunk of steps."""
+    kernel's schedule and tile shape (od_batch_tune); the choice
+    (sym_chunk, sym_tile) is returned, and results are reproducible for it
+    (pass it back as sym_chunk and sym_tile).  progress(done, total) is called
+    after every chunk of steps."""
     if len(openDusts) == 0:
         return
     dev_str = str(deviceIndex).strip()
@@ -89,6 +90,8 @@ def simulateBatch(openDusts, deviceIndex="0", useGraph=True, alphas=None, precis
     cfgs, states = prepare_batch(openDusts, device, alphas, precision)
     if sym_chunk is not None:
         cfgs[0].sym_chunk = int(sym_chunk)
+    if sym_tile is not None:
+        cfgs[0].sym_tile = int(sym_tile)
     batch = Batch(cfgs)
     try:
         for s, (x0, v0) in enumerate(states):

@@ -69,6 +69,24 @@ def test_invalid_config_is_an_error_not_a_crash(eng):
     assert rc == 1 and b"layout" in eng.load_library().od_last_error()
 
 
+def test_sym_variant_table_and_range(eng):
+    v = eng.sym_variants()
+    assert v[0] == (8, 3) and len(v) >= 2
+    assert all(mt in (4, 8, 16) and 1 <= w <= 8 for mt, w in v)
+    lib = eng.load_library()
+    mt = ctypes.c_int32()
+    assert lib.od_sym_variant_info(len(v) + 1, ctypes.byref(mt), None) == 1
+    assert b"variant" in lib.od_last_error()
+    cfg = eng.Config()
+    cfg.abi_version = eng.ABI_VERSION
+    cfg.struct_size = ctypes.sizeof(eng.Config)
+    cfg.n_ions = 16
+    cfg.sym_tile = len(v) + 1
+    ctx = ctypes.c_void_p()
+    assert lib.od_create(ctypes.byref(cfg), ctypes.byref(ctx)) == 1 and not ctx
+    assert b"sym_tile" in lib.od_last_error()
+
+
 def test_engine_host_sampler_matches_oracle(eng):
     from oracle import oracle
 

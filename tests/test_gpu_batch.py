@@ -167,7 +167,7 @@ def test_simulate_batch_api_vs_oracle(oracle_lib):
 
 def test_batch_tune_is_reproducible():
     """The autotuner's choice fixes the schedule: a tuned batch and a batch
-    created with that sym_chunk give bitwise identical steps."""
+    created with that sym_chunk and sym_tile give bitwise identical steps."""
     from opendust_amd import engine
 
     ods, cfgs, states = _batch_setup("fp32c")
@@ -176,18 +176,19 @@ def test_batch_tune_is_reproducible():
     chosen = None
     for tune in (True, False):
         if not tune:
-            cfgs[0].sym_chunk = chosen
+            cfgs[0].sym_chunk, cfgs[0].sym_tile = chosen
         b = engine.Batch(cfgs)
         for s, (x0, v0) in enumerate(states):
             b.set_ions(s, x0, v0)
         if tune:
             chosen = b.tune(2)
-            say("  chosen sym_chunk", chosen)
-            assert chosen in (1, 2, 4, 8, 16, 32)
+            say("  chosen sym_chunk, sym_tile", chosen)
+            assert chosen[0] in (1, 2, 4, 8, 16, 32)
+            assert 1 <= chosen[1] <= len(engine.sym_variants())
         rec = b.run(12, engine.RUN_GRAPH)
         outs.append((rec, [b.get_ions(s) for s in range(len(cfgs))]))
         b.close()
-    cfgs[0].sym_chunk = 0
+    cfgs[0].sym_chunk = cfgs[0].sym_tile = 0
     assert np.array_equal(outs[0][0], outs[1][0])
     for (x, v), (x1, v1) in zip(outs[0][1], outs[1][1]):
         assert np.array_equal(x, x1) and np.array_equal(v, v1)

@@ -91,6 +91,36 @@ def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
+    """Every variant of the symmetric kernel the autotuner may pick (tile
+    shape, waves per SIMD) meets the FP32c force bound against the oracle, for
+    one and for several source blocks per work-group."""
+    eng = _engine_mod()
+    variants = eng.sym_variants()
+    if tile > len(variants):
+        pytest.skip("library has %d variants" % len(variants))
+    od, cfg, x0, v0 = _setup(N=5000, precision="fp32c", craft=False, mode="sym")
+    f_orc = oracle_lib.pair_forces(cfg, x0)
+    fn = np.linalg.norm(f_orc, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    for ch in (1, 2):
+        cfg.sym_tile, cfg.sym_chunk = tile, ch
+        say("sym variant", tile, variants[tile - 1], "chunk", ch, "N", cfg.n_ions)
+        e = eng.Engine(cfg)
+        assert e.pair_info()[0] == MODES["sym"]
+        f_gpu = e.pair_forces(x0)
+        e.close()
+        err = np.linalg.norm(f_gpu - f_orc, axis=1)
+        say("  norm-wise rel", np.linalg.norm(f_gpu - f_orc) / np.linalg.norm(f_orc))
+        assert np.linalg.norm(f_gpu - f_orc) <= 1e-5 * np.linalg.norm(f_orc)
+        assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
+    bad = eng.Config.from_buffer_copy(cfg)
+    bad.sym_tile = len(variants) + 1
+    with pytest.raises(RuntimeError, match="sym_tile"):
+        eng.Engine(bad)
+
+
 @pytest.mark.parametrize("precision,mode,workload", [
     ("fp64", "rows", "single_dust"), ("fp32c", "rows", "single_dust"), ("fp32c", "sym", "single_dust"),
     ("fp64", "rows", "dust_chain"), ("fp32c", "sym", "dust_chain")])
@@ -170,8 +200,9 @@ def test_determinism_and_graph(oracle_lib, mode):
 
 
 def test_tune_is_reproducible():
-    """od_tune picks the symmetric kernel's source blocks per work-group; a
-    context created with that choice reproduces the tuned run bitwise."""
+    """od_tune picks the symmetric kernel's source blocks per work-group and
+    kernel variant; a context created with that choice reproduces the tuned
+    run bitwise."""
     eng = _engine_mod()
     od, cfg, x0, v0 = _setup(N=5000, precision="fp32c", craft=True, mode="sym")
     say("tune fp32c sym N", cfg.n_ions)
@@ -179,13 +210,14 @@ def test_tune_is_reproducible():
     chosen = None
     for tune in (True, False):
         if not tune:
-            cfg.sym_chunk = chosen
+            cfg.sym_chunk, cfg.sym_tile = chosen
         e = eng.Engine(cfg)
         e.set_ions(x0, v0)
         if tune:
             chosen = e.tune(2)
-            say("  chosen sym_chunk", chosen)
-            assert chosen in (1, 2, 4, 8, 16, 32)
+            say("  chosen sym_chunk, sym_tile", chosen)
+            assert chosen[0] in (1, 2, 4, 8, 16, 32)
+            assert 1 <= chosen[1] <= len(eng.sym_variants())
         rec = e.run(12, eng.RUN_GRAPH)
         outs.append((rec, e.get_ions()))
         e.close()
