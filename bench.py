@@ -30,10 +30,14 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector with packed ops; also matrix), TFLOP/s
 FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
-# HBM bytes per k_pair32_sym launch at 2^20 ions for a given number of source
-# blocks per work-group, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a
-# 1 GiB copy's readings (x2.000, x1.000): profiles/k_pair32_sym_2e20_pmc.md
-SYM_TRAFFIC_2E20 = {16: 13.873e9, 1: 27.430e9}
+# HBM bytes per k_pair32_sym launch at 2^20 ions for a given kernel variant and
+# number of source blocks per work-group, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# scaled by a 1 GiB copy's readings (x2.000, x1.000): profiles/k_pair32_sym_2e20_pmc.md
+# (variants 2 and 4 spill registers; their scratch traffic is included)
+SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes per launch
+    (1, 16): 13.873e9, (1, 1): 27.430e9, (1, 2): 20.442e9, (2, 1): 43.531e9, (2, 2): 30.513e9,
+    (3, 1): 27.449e9, (3, 2): 20.549e9, (4, 1): 31.020e9, (4, 2): 22.758e9,
+}
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
 
 
@@ -331,10 +335,10 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            # profiled per schedule with kernel variant 1 (the untuned default is 16 source blocks per
-            # work-group at 2^20)
-            "traffic": SYM_TRAFFIC_2E20.get(chunk or 16) if (mode == engine.PAIR_SYM and N == 2 ** 20
-                                                              and tile == 1) else None,
+            # profiled per (variant, schedule); the untuned default is variant 1 with 16 source blocks per
+            # work-group at 2^20
+            "traffic": SYM_TRAFFIC_2E20.get((tile, chunk or 16)) if (mode == engine.PAIR_SYM and N == 2 ** 20)
+            else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
