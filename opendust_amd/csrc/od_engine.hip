@@ -2115,6 +2115,16 @@ static int sym_build(C* c, const std::vector<int>& be, int ch) {
   return OD_OK;
 }
 
+// default kernel variant: the configured one, else 8 target sub-blocks per
+// wave (2 waves per work-group) when the schedule has at least 2048
+// work-groups (4 waves per SIMD), else 4 (4 waves per work-group: at 2^15 ions
+// the 496 work-groups then still give about 2 waves per SIMD).  Measured on
+// MI355X at 2^15 ions: 0.33 ms per evaluation with 4, 0.64 ms with 8.
+static int sym_default_variant(int64_t n_wg, int configured) {
+  if (configured > 0) return configured - 1;
+  return n_wg >= 2048 ? 0 : 2;
+}
+
 // default source blocks per work-group: the largest of 16, 8, .. that still
 // gives >= 4096 work-groups (8 per XCD CU set), or the configured value
 static int sym_default_chunk(const std::vector<int>& be, int configured) {
@@ -2158,13 +2168,15 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   if (c->prec != OD_PREC_FP32C || cfg->pair_mode == OD_PAIR_ROWS || cfg->cutoff > 0.0) return OD_OK;
   const int T = c->n_pad / SYM_TB;
   const bool forced = cfg->pair_mode == OD_PAIR_SYM;
-  if (T < 2 || (!forced && c->n_pad < (1 << 17))) return OD_OK;
+  // automatic choice: symmetric from 2^15 ions (measured on MI355X, one step:
+  // 2^15 0.47 ms symmetric vs 0.62 ms rows, 2^14 0.30 vs 0.22 ms)
+  if (T < 2 || (!forced && c->n_pad < (1 << 15))) return OD_OK;
   const std::vector<int> be(T, T);
   const int ch = sym_default_chunk(be, cfg->sym_chunk);
   if (!forced && sym_bytes(be, ch) > 48e9) return OD_OK;
   int rc = OD_OK;
   if ((rc = sym_build(c, be, ch)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8))) return rc;
-  c->sym_var = cfg->sym_tile > 0 ? cfg->sym_tile - 1 : 0;
+  c->sym_var = sym_default_variant(c->n_wg, cfg->sym_tile);
   c->sym = 1;
   return OD_OK;
 }
@@ -3046,7 +3058,6 @@ extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch**
   }
   b->n_items = (int)items.size();
   b->sym = (b->prec == OD_PREC_FP32C && cfgs[0].pair_mode != OD_PAIR_ROWS) ? 1 : 0;
-  b->sym_var = cfgs[0].sym_tile > 0 ? cfgs[0].sym_tile - 1 : 0;
   const size_t np = b->n_tot;
   int rc = OD_OK;
   if (hipSetDevice(b->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice failed"));
@@ -3120,6 +3131,7 @@ extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch**
         hipMemcpy(b->blk_a, ba.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->blk_scale, bs.data(), NB * 8, hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(OD_ERR_HIP, "batch initialisation failed"));
+    b->sym_var = sym_default_variant(b->n_wg, cfgs[0].sym_tile);
   }
   *out = b;
   return OD_OK;

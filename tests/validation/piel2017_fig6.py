@@ -1,0 +1,100 @@
+"""Piel 2017, Fig. 6 (examples/Piel2017/Figure6/launchPiel2017Figure6.py): the
+reference's 7-point Mach scan of the forces on two dust grains, run here as
+batched sweeps (opendust_amd.sweep.simulateBatch) and compared point by point
+with the means the reference printed (tests/golden/piel2017_fig6_reference.txt).
+
+Each point: Ar+, T_e = 29011 K, T_n = 290.11 K, n = 1e14 m^-3, Mach number M
+from the scan, R = 3, H = 6 r_D_e, 2^17 superions, 50000 steps of 1e-9 s; two
+grains of radius 12 um with given charge 40939.491598 e at (0, 0, -0.5 r_D_e)
+and (0.5 r_D_e, 0, 0.5 r_D_e).  Reported per point: the means over steps
+i > 10000 of the x and z forces (orbit + dust-dust) on both grains, with the
+standard error from 10 block averages, and z = (ours - reference) / (sqrt(2) sigma).
+
+    python tests/validation/piel2017_fig6.py [--out results.json] [--points 0,1,...] [--steps 50000]
+
+Runs on the GPU (three points in one batch take about 14 minutes on one MI355X).
+"""
+
+import argparse
+import contextlib
+import io
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+MACH = [0.29940, 0.40129, 0.50033, 0.60133, 0.70098, 0.80019, 1.00091]
+
+
+def build(points, steps, seed=61):
+    from opendust_amd.opendust import (DustParticle, OpenDust, OutputParameters, PlasmaParametersInSIUnitsMaxwell,
+                                       SimulationParametersInSIUnits)
+
+    ods = []
+    for p in points:
+        pp = PlasmaParametersInSIUnitsMaxwell(290.11, 29011, 1e14, MACH[p], 6.6335209e-26)
+        lam = pp.r_D_e
+        sp = SimulationParametersInSIUnits(3 * lam, 6 * lam, int(2 ** 17), steps, 1e-09, pp)
+        q = 40939.491598 * pp.e
+        d1 = DustParticle(0, 0, -0.5 * lam, 12.0e-6, "given", q)
+        d2 = DustParticle(0.5 * lam, 0, 0.5 * lam, 12.0e-6, "given", q)
+        ods.append(OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), [d1, d2], "Maxwellian", precision="fp32c",
+                            seed=seed + p, trapSeed=6))
+    return ods
+
+
+def block_mean(a, n_blocks=10):
+    a = np.asarray(a, dtype=np.float64)
+    k = len(a) // n_blocks
+    blocks = a[len(a) - k * n_blocks:].reshape(n_blocks, k).mean(axis=1)
+    return float(a.mean()), float(blocks.std(ddof=1) / np.sqrt(n_blocks))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="")
+    ap.add_argument("--points", default=",".join(str(i) for i in range(7)))
+    ap.add_argument("--steps", type=int, default=50000)
+    args = ap.parse_args()
+    points = [int(t) for t in args.points.split(",")]
+    ref = np.loadtxt(os.path.join(os.path.dirname(HERE), "golden", "piel2017_fig6_reference.txt"))
+    from opendust_amd import sweep
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        ods = build(points, args.steps)
+    np.random.seed(2017 + points[0])
+    t0 = time.time()
+    print("points %s, %d steps" % (points, args.steps), flush=True)
+    sweep.simulateBatch(ods, "0", tune=True,
+                        progress=lambda i, n: print("  step %d of %d, %.0f s" % (i, n, time.time() - t0), flush=True))
+    wall = time.time() - t0
+    start = min(10001, args.steps // 5)
+    rows = []
+    for od, p in zip(ods, points):
+        d1, d2 = od.dustParticles
+        f = [d1.forceIonsOrbit[:, 0] + d1.forceDust[:, 0], d1.forceIonsOrbit[:, 2] + d1.forceDust[:, 2],
+             d2.forceIonsOrbit[:, 0] + d2.forceDust[:, 0], d2.forceIonsOrbit[:, 2] + d2.forceDust[:, 2]]
+        row = {"M": MACH[p]}
+        for name, series, r in zip(("F1x", "F1z", "F2x", "F2z"), f, ref[p, 1:]):
+            m, s = block_mean(series[start:])
+            row[name] = [m, s, float(r), float((m - r) / (np.sqrt(2.0) * s))]
+        rows.append(row)
+        print("M %.5f  " % MACH[p] + "  ".join("%s %.3e+-%.1e (ref %.3e, z %+.2f)" % (k, *row[k]) for k in
+                                              ("F1x", "F1z", "F2x", "F2z")), flush=True)
+    zs = np.array([row[k][3] for row in rows for k in ("F1x", "F1z", "F2x", "F2z")])
+    summary = {"points": points, "steps": args.steps, "average_from_step": start, "wall_s": wall,
+               "max_abs_z": float(np.max(np.abs(zs))), "rms_z": float(np.sqrt(np.mean(zs ** 2))), "rows": rows}
+    print("max |z| %.2f, rms z %.2f over %d values; batched run %.1f s" % (summary["max_abs_z"], summary["rms_z"],
+                                                                        zs.size, wall), flush=True)
+    if args.out:
+        with open(args.out, "w") as fh:
+            json.dump(summary, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
