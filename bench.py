@@ -35,8 +35,10 @@ FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
 # scaled by a 1 GiB copy's readings (x2.000, x1.000): profiles/k_pair32_sym_2e20_pmc.md
 # (variants 2 and 4 spill registers; their scratch traffic is included)
 SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes per launch
-    (1, 16): 13.873e9, (1, 1): 27.430e9, (1, 2): 20.442e9, (2, 1): 43.531e9, (2, 2): 30.513e9,
-    (3, 1): 27.449e9, (3, 2): 20.549e9, (4, 1): 31.020e9, (4, 2): 22.758e9,
+    (1, 1): 27.430e9, (1, 2): 20.442e9, (1, 4): 16.905e9, (1, 8): 15.002e9, (1, 16): 13.873e9, (1, 32): 13.267e9,
+    (2, 1): 43.531e9, (2, 2): 30.513e9, (2, 4): 23.763e9, (2, 8): 20.361e9, (2, 16): 18.394e9, (2, 32): 17.445e9,
+    (3, 1): 27.449e9, (3, 2): 20.549e9, (3, 4): 17.092e9, (3, 8): 15.010e9, (3, 16): 13.793e9, (3, 32): 12.922e9,
+    (4, 1): 31.020e9, (4, 2): 22.758e9, (4, 4): 18.547e9, (4, 8): 16.588e9, (4, 16): 15.480e9, (4, 32): 14.710e9,
 }
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
 
