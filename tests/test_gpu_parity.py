@@ -20,6 +20,9 @@ Tolerances (stated here, checked below):
     bit-exact.
 """
 
+import contextlib
+import io
+
 import numpy as np
 import pytest
 
@@ -121,6 +124,58 @@ def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
         eng.Engine(bad)
 
 
+def test_full_size_sym_forces_vs_fp64_gpu():
+    """At the benchmark size, 2^20 ions, the FP32c symmetric kernel against the
+    plain FP64 row kernel on the GPU (independent code: ordered pairs, FP64
+    law, no partials), for the default variant and schedule and for 4 targets
+    per lane with 16 source blocks per work-group.
+
+    Two error sources are separated.  (a) Kernel arithmetic: against FP64 on
+    the same positions rounded to FP32 in the kernel's units (what the FP32
+    kernel sees), every pair term carries a few FP32 roundings (2^-24 each)
+    and the sums are compensated, so per ion |dF_i| <= 1e-5 |F_i| + 1e-6
+    rms|F| and norm-wise <= 2e-6: every block pair of the triangular schedule
+    is present and every partial lands on its ions.  (b) Position
+    quantization: against FP64 on the exact positions the rounding of the
+    coordinates (ulp 4.8e-7 in units of r_D_e/log2 e at |u| ~ 7) moves a pair
+    at distance r by ~ulp/r relatively; at this density (nearest neighbours
+    ~0.05 r_D_e, some far closer) that dominates: norm-wise <= 1e-4 (measured
+    ~4e-5), per ion not bounded by the small-N model.  (The CPU oracle would
+    need ~10 minutes at this size; it pins both kernels at small sizes.)"""
+    eng = _engine_mod()
+    from opendust_amd import workloads
+
+    od = workloads.single_dust(N=2 ** 20, n=1, precision="fp64", seed=3)
+    alpha = workloads.synthetic_alpha(od)
+    cfg, x0, v0 = workloads.prepare(od, alpha=alpha, np_seed=5, precision="fp64")
+    scale = 1.4426950408889634 / cfg.r_D_e
+    x32 = (x0 * scale).astype(np.float32).astype(np.float64) / scale
+    say("full size: fp64 rows N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    f64 = e.pair_forces(x0)
+    f64r = e.pair_forces(x32)
+    e.close()
+    cfg32, _, _ = workloads.prepare(od, alpha=alpha, np_seed=5, precision="fp32c")
+    for tile, ch in ((0, 0), (3, 16)):
+        cfg32.sym_tile, cfg32.sym_chunk = tile, ch
+        e = eng.Engine(cfg32)
+        assert e.pair_info()[0] == MODES["sym"]
+        f32 = e.pair_forces(x0)
+        e.close()
+        fn = np.linalg.norm(f64r, axis=1)
+        rms = np.sqrt(np.mean(fn ** 2))
+        err = np.linalg.norm(f32 - f64r, axis=1)
+        rel_a = np.linalg.norm(f32 - f64r) / np.linalg.norm(f64r)
+        rel_b = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
+        rel_q = np.linalg.norm(f64r - f64) / np.linalg.norm(f64)
+        say("  fp32c sym tile", tile, "chunk", ch, "| arithmetic: norm-wise", rel_a, "worst per-ion err / bound",
+            np.max(err / (1e-5 * fn + 1e-6 * rms)), "| vs exact positions: norm-wise", rel_b,
+            "(fp64 on rounded positions alone:", rel_q, ")")
+        assert rel_a <= 2e-6
+        assert np.all(err <= 1e-5 * fn + 1e-6 * rms)
+        assert rel_b <= 1e-4
+
+
 @pytest.mark.parametrize("precision,mode,workload", [
     ("fp64", "rows", "single_dust"), ("fp32c", "rows", "single_dust"), ("fp32c", "sym", "single_dust"),
     ("fp64", "rows", "dust_chain"), ("fp32c", "sym", "dust_chain")])
@@ -178,6 +233,66 @@ def test_multi_step_fp64_records(oracle_lib, workload):
         sc = np.max(np.abs(rec_o[key]))
         assert np.max(np.abs(rec[key] - rec_o[key])) <= 1e-9 * sc, key
     assert np.max(np.abs(x_g - x_o)) <= 1e-10 * cfg.R
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_four_dusts_mixed_charging_vs_oracle(oracle_lib, precision):
+    """The most dusts a configuration holds (4), off the axis, one per charge
+    method plus a second consistent one, ions crafted inside every dust: six
+    steps against the oracle (FP64: the multi-step tolerances; FP32c: counts
+    and given charges exact, forces and positions within the FP32c bounds of
+    one step, then counts over all six steps)."""
+    eng = _engine_mod()
+    from opendust_amd.opendust import (DustParticle, OpenDust, OutputParameters, PlasmaParametersInSIUnitsMaxwell,
+                                       SimulationParametersInSIUnits)
+    from opendust_amd import workloads
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        pp = PlasmaParametersInSIUnitsMaxwell(290.11, 29011, 3.57e14, 1.3, 6.63e-26)
+        lam = pp.r_D_e
+        sp = SimulationParametersInSIUnits(4 * lam, 8 * lam, 3000, 6, 0.01 / pp.w_p_i, pp)
+        e_ = abs(pp.e)
+        dusts = [DustParticle(0.3 * lam, 0.0, -1.5 * lam, 0.08 * lam, "consistent", -2e4 * e_),
+                 DustParticle(-0.2 * lam, 0.4 * lam, -0.4 * lam, 0.06 * lam, "oml", -1.5e4 * e_),
+                 DustParticle(0.0, -0.5 * lam, 0.7 * lam, 0.07 * lam, "given", -3e4 * e_),
+                 DustParticle(0.5 * lam, 0.5 * lam, 1.8 * lam, 0.05 * lam, "consistent", -1e4 * e_)]
+        od = OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dusts, "Maxwellian", precision=precision, seed=29)
+    cfg, x0, v0 = workloads.prepare(od, alpha=workloads.synthetic_alpha(od), np_seed=8, precision=precision)
+    assert cfg.n_dust == 4
+    rng = np.random.RandomState(3)
+    for i in range(8):
+        p = i % 4
+        d = rng.normal(size=3)
+        d *= 0.3 * cfg.dust_r[p] / np.linalg.norm(d)
+        x0[i] = [cfg.dust_x[p] + d[0], cfg.dust_y[p] + d[1], cfg.dust_z[p] + d[2]]
+    q0 = np.array([cfg.dust_q0[p] for p in range(4)])
+    say("four dusts", precision, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    rec1 = e.run(1)
+    x1, _ = e.get_ions()
+    rec = e.run(5)
+    e.close()
+    x_o1, _, _, rec_o1, fl_o1, _ = oracle_lib.step(cfg, x0, v0, q0, 0)
+    assert np.all(rec1["absorbed"][0] >= 2), rec1["absorbed"]
+    assert np.array_equal(rec1["absorbed"][0], rec_o1["absorbed"])
+    assert rec1["q"][0, 2] == q0[2]
+    if precision == "fp64":
+        x_o, v_o, q_o, rec_o = oracle_lib.run(cfg, x0, v0, q0, 0, 6)
+        rec_all = np.concatenate([rec1, rec])
+        assert np.array_equal(rec_all["absorbed"], rec_o["absorbed"])
+        assert np.max(np.abs(rec_all["q"] - rec_o["q"]) / np.abs(rec_o["q"])) <= 1e-12
+        for key in ("force_orbit", "force_collect", "force_dust"):
+            sc = np.max(np.abs(rec_o[key]))
+            assert np.max(np.abs(rec_all[key] - rec_o[key])) <= 1e-9 * sc, key
+    else:
+        ok, worst, bound = fp32c_position_error_ok(oracle_lib, cfg, x0, x1, x_o1, fl_o1)
+        say("  worst |dx_i| / bound_i", worst)
+        assert ok
+        ratios = check_records(cfg, rec1[0], rec_o1, precision, x_o1, bound)
+        say("  records, error / tolerance:", ratios)
+        for key, r in ratios.items():
+            assert r <= 1.0, key
 
 
 @pytest.mark.parametrize("mode", ["rows", "sym"])
