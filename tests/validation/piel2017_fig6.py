@@ -15,6 +15,9 @@ for both, and the 100-step block means (for later analysis).
 
     python tests/validation/piel2017_fig6.py [--out results.json] [--points 0,1,...] [--steps 50000]
 
+A point listed more than once (--points 3,3,3) runs independent realisations,
+which measure the run-to-run spread directly.
+
 Runs on the GPU (three points in one batch take about 14 minutes on one MI355X).
 """
 
@@ -35,11 +38,14 @@ MACH = [0.29940, 0.40129, 0.50033, 0.60133, 0.70098, 0.80019, 1.00091]
 
 
 def build(points, steps, seed=61):
+    """One OpenDust per entry of `points`; a point listed again gets an
+    independent realisation (Philox key, trap seed)."""
     from opendust_amd.opendust import (DustParticle, OpenDust, OutputParameters, PlasmaParametersInSIUnitsMaxwell,
                                        SimulationParametersInSIUnits)
 
     ods = []
-    for p in points:
+    for i, p in enumerate(points):
+        rep = points[:i].count(p)
         pp = PlasmaParametersInSIUnitsMaxwell(290.11, 29011, 1e14, MACH[p], 6.6335209e-26)
         lam = pp.r_D_e
         sp = SimulationParametersInSIUnits(3 * lam, 6 * lam, int(2 ** 17), steps, 1e-09, pp)
@@ -47,7 +53,7 @@ def build(points, steps, seed=61):
         d1 = DustParticle(0, 0, -0.5 * lam, 12.0e-6, "given", q)
         d2 = DustParticle(0.5 * lam, 0, 0.5 * lam, 12.0e-6, "given", q)
         ods.append(OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), [d1, d2], "Maxwellian", precision="fp32c",
-                            seed=seed + p, trapSeed=6))
+                            seed=seed + p + 1000 * rep, trapSeed=6 + rep))
     return ods
 
 
@@ -95,7 +101,7 @@ def main():
         d1, d2 = od.dustParticles
         f = [d1.forceIonsOrbit[:, 0] + d1.forceDust[:, 0], d1.forceIonsOrbit[:, 2] + d1.forceDust[:, 2],
              d2.forceIonsOrbit[:, 0] + d2.forceDust[:, 0], d2.forceIonsOrbit[:, 2] + d2.forceDust[:, 2]]
-        row = {"M": MACH[p]}
+        row = {"M": MACH[p], "realisation": points[:len(rows)].count(p)}
         for name, series, r in zip(("F1x", "F1z", "F2x", "F2z"), f, ref[p, 1:]):
             m, s = block_mean(series[start:])
             sb = blocking_error(series[start:])
