@@ -267,3 +267,30 @@ def test_dust_langevin_step_drag_gravity_and_noise():
                    for _ in range(4000)])
     var_expect = 2 * 1.380649e-23 * T * m_p * gamma / dt * (dt / m_p) ** 2
     assert abs(dv.var() / var_expect - 1) < 0.06
+
+
+def test_validation_fixtures_match_the_scan_tables():
+    """The reference means the GPU validation scripts compare with parse, and
+    their parameter columns are the scans the scripts run (Mach numbers, d_x,
+    collision frequencies as the reference printed them)."""
+    import importlib.util
+
+    here = os.path.dirname(__file__)
+
+    def script(name):
+        spec = importlib.util.spec_from_file_location(name, os.path.join(here, "validation", name + ".py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        return mod
+
+    golden = os.path.join(here, "golden")
+    f6 = np.loadtxt(os.path.join(golden, "piel2017_fig6_reference.txt"))
+    assert f6.shape == (7, 5) and np.allclose(f6[:, 0], script("piel2017_fig6").MACH, rtol=0, atol=1e-5)
+    f7 = np.loadtxt(os.path.join(golden, "piel2017_fig7_reference.txt"))
+    assert f7.shape == (11, 5) and np.allclose(f7[:, 0], script("piel2017_fig7").D_X, rtol=0, atol=1e-5)
+    pt = np.loadtxt(os.path.join(golden, "patacchini2008_fig3_reference.txt"))
+    s = script("patacchini2008_fig3")
+    assert pt.shape == (13, 4) and np.allclose(pt[:, 0], s.W_C, rtol=1e-8, atol=0)
+    # the field gives Mach 1 at every collision frequency (v_fl = e E / (m_i w_c))
+    assert np.allclose(np.array(s.E_FIELD) / np.array(s.W_C), s.E_FIELD[0] / s.W_C[0], rtol=1e-6)
+    assert np.all(pt[:, 1] < 0) and np.all(np.abs(pt[:, 3]) > 0)
