@@ -203,7 +203,7 @@ def main():
                          "with --no-tune)")
     ap.add_argument("--sym-tile", type=int, default=0,
                     help="symmetric kernel variant (1-based; tile shape and waves per SIMD); 0: autotune, or "
-                         "variant 1 with --no-tune or --sym-chunk")
+                         "the size-dependent default with --no-tune or --sym-chunk")
     ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
     args = ap.parse_args()
 
@@ -231,11 +231,11 @@ def main():
     eng = engine.Engine(cfg)
     eng.set_ions(x0, v0)
     mode, pairs_launch = eng.pair_info()
-    chunk, tile = args.sym_chunk, max(args.sym_tile, 1)
     if not args.no_tune and not args.pair_only and not args.sym_chunk and not args.sym_tile:
         chunk, tile = eng.tune(2)
         log("autotuner chose %d source blocks per work-group, kernel variant %d %s"
             % (chunk, tile, engine.sym_variants()[tile - 1]))
+    chunk, tile = eng.sym_choice()
 
     if args.pair_only:
         for _ in range(args.pair_only):
@@ -339,8 +339,7 @@ def main():
             "frac": achieved / FP32_PEAK_TFLOPS,
             # profiled per (variant, schedule); the untuned default is variant 1 with 16 source blocks per
             # work-group at 2^20
-            "traffic": SYM_TRAFFIC_2E20.get((tile, chunk or 16)) if (mode == engine.PAIR_SYM and N == 2 ** 20)
-            else None,
+            "traffic": SYM_TRAFFIC_2E20.get((tile, chunk)) if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,
@@ -348,7 +347,8 @@ def main():
             "pairs_per_launch": pairs_launch,
             "sym_chunk": chunk,
             "sym_tile": tile,
-            "sym_tile_shape": dict(zip(("targets_per_lane", "waves_per_simd"), engine.sym_variants()[tile - 1])),
+            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd"), engine.sym_variants()[tile - 1]))
+                               if tile else None),
         },
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,

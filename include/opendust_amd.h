@@ -205,6 +205,11 @@ int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t res
  * symmetric kernel.  No counterpart in the reference (measurement support). */
 int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
+/* The symmetric kernel's current schedule: source blocks per work-group and
+ * kernel variant (1-based), as configured, defaulted or tuned; 0 and 0 when
+ * the context does not run it.  Measurement support. */
+int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile);
+
 /* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
  * return *sym_chunk = *sym_tile = 0 and change nothing).  It times the whole
  * pair evaluation `reps` times per candidate on the current ions: first 1, 2,

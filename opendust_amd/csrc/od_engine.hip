@@ -2706,6 +2706,13 @@ extern "C" int od_cutoff_pair_count(od_ctx* c, int64_t* in_range, int64_t* candi
   return OD_OK;
 }
 
+extern "C" int od_sym_choice(od_ctx* c, int32_t* sym_chunk, int32_t* sym_tile) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (sym_chunk) *sym_chunk = c->sym ? c->sym_ch : 0;
+  if (sym_tile) *sym_tile = c->sym ? c->sym_var + 1 : 0;
+  return OD_OK;
+}
+
 extern "C" int od_pair_info(od_ctx* c, int32_t* mode, int64_t* pairs_per_launch) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   const int64_t n = c->n;

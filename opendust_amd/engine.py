@@ -145,6 +145,7 @@ EXPORTS = [
     "od_last_error", "od_build_info",
     "od_batch_create", "od_batch_set_ions", "od_batch_get_ions", "od_batch_run", "od_batch_get_step_info",
     "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune", "od_sym_variants", "od_sym_variant_info",
+    "od_sym_choice",
     "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
     "od_update_config",
 ]
@@ -191,6 +192,7 @@ def load_library(path=LIB_PATH):
     lib.od_batch_destroy.argtypes = [c_ctx]
     lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_sym_variants.argtypes = []
+    lib.od_sym_choice.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_update_config.argtypes = [c_ctx, P(Config)]
     lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
@@ -417,6 +419,13 @@ class Engine:
         ch = ctypes.c_int32()
         tile = ctypes.c_int32()
         _check(self.lib.od_tune(self._ctx, int(reps), ctypes.byref(ch), ctypes.byref(tile)), "od_tune")
+        return ch.value, tile.value
+
+    def sym_choice(self):
+        """(source blocks per work-group, kernel variant) the symmetric kernel
+        runs with now; (0, 0) when the context does not run it."""
+        ch, tile = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.od_sym_choice(self._ctx, ctypes.byref(ch), ctypes.byref(tile)), "od_sym_choice")
         return ch.value, tile.value
 
     def pair_forces(self, x):

@@ -333,6 +333,7 @@ def test_tune_is_reproducible():
             say("  chosen sym_chunk, sym_tile", chosen)
             assert chosen[0] in (1, 2, 4, 8, 16, 32)
             assert 1 <= chosen[1] <= len(eng.sym_variants())
+        assert e.sym_choice() == chosen
         rec = e.run(12, eng.RUN_GRAPH)
         outs.append((rec, e.get_ions()))
         e.close()
