@@ -3361,6 +3361,13 @@ template <class C, class F>
 static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen_ch, int* chosen_var) {
   double best_ms = 1e300;
   int best_ch = c->sym_ch, best_var = c->sym_var;
+  // a captured step holds the schedule's buffers, which sym_build replaces:
+  // drop it before the first rebuild, so no error path can leave it behind
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
   auto time_one = [&](int ch, int var) -> int {
     if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > 48e9) return OD_OK;
     if (c->sym_ch != ch) CHKRET(sym_build(c, be, ch));
@@ -3391,11 +3398,6 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
   }
   if (c->sym_ch != best_ch) CHKRET(sym_build(c, be, best_ch));
   c->sym_var = best_var;
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
-  c->graph_steps = 0;
   *chosen_ch = best_ch;
   *chosen_var = best_var;
   return OD_OK;
