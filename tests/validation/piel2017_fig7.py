@@ -12,6 +12,9 @@ error from 10 block averages, and z = (ours - reference) / (sqrt(2) sigma).
 
     python tests/validation/piel2017_fig7.py [--out results.json] [--points 0,1,...]
 
+A point listed more than once (--points 5,5,5) runs independent realisations,
+which measure the run-to-run spread directly.
+
 Runs on the GPU (about 5 minutes for all 11 points on one MI355X).
 """
 
@@ -32,11 +35,14 @@ D_X = [0.00176, 0.10369, 0.20211, 0.30053, 0.40246, 0.50088, 0.60105, 0.70123, 0
 
 
 def build(points, seed=17):
+    """One OpenDust per entry of `points`; a point listed again gets an
+    independent realisation (Philox key, trap seed)."""
     from opendust_amd.opendust import (DustParticle, OpenDust, OutputParameters, PlasmaParametersInSIUnitsMaxwell,
                                        SimulationParametersInSIUnits)
 
     ods = []
     for k, p in enumerate(points):
+        rep = points[:k].count(p)
         pp = PlasmaParametersInSIUnitsMaxwell(290.11, 29011, 1e14, 1, 6.6335209e-26)
         lam = pp.r_D_e
         sp = SimulationParametersInSIUnits(3 * lam, 6 * lam, int(2 ** 17), 5000, 0.5148240854e-09, pp)
@@ -44,7 +50,7 @@ def build(points, seed=17):
         d1 = DustParticle(0, 0, -0.5 * lam, 25.0e-6, "given", q)
         d2 = DustParticle(D_X[p] * lam, 0, 0.5 * lam, 25.0e-6, "given", q)
         ods.append(OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), [d1, d2], "Maxwellian", precision="fp32c",
-                            seed=seed + k, trapSeed=5))
+                            seed=seed + p + 1000 * rep, trapSeed=5 + rep))
     return ods
 
 
@@ -53,6 +59,19 @@ def block_mean(a, n_blocks=10):
     k = len(a) // n_blocks
     blocks = a[len(a) - k * n_blocks:].reshape(n_blocks, k).mean(axis=1)
     return float(a.mean()), float(blocks.std(ddof=1) / np.sqrt(n_blocks))
+
+
+def blocking_error(a, min_blocks=8):
+    """Standard error of the mean of a correlated series by repeated pairwise
+    blocking (Flyvbjerg and Petersen, J. Chem. Phys. 91, 461 (1989)): the
+    largest estimate over the blocking levels that keep >= min_blocks blocks."""
+    x = np.asarray(a, dtype=np.float64)
+    best = 0.0
+    while len(x) >= min_blocks:
+        best = max(best, float(x.std(ddof=1) / np.sqrt(len(x))))
+        n = len(x) // 2
+        x = 0.5 * (x[0:2 * n:2] + x[1:2 * n:2])
+    return best
 
 
 def main():
@@ -76,13 +95,15 @@ def main():
         d1, d2 = od.dustParticles
         f = [d1.forceIonsOrbit[:, 0] + d1.forceDust[:, 0], d1.forceIonsOrbit[:, 2] + d1.forceDust[:, 2],
              d2.forceIonsOrbit[:, 0] + d2.forceDust[:, 0], d2.forceIonsOrbit[:, 2] + d2.forceDust[:, 2]]
-        row = {"d_x": D_X[p]}
+        row = {"d_x": D_X[p], "realisation": points[:len(rows)].count(p)}
         for name, series, r in zip(("F1x", "F1z", "F2x", "F2z"), f, ref[p, 1:]):
             m, s = block_mean(series[2501:])
-            row[name] = [m, s, float(r), float((m - r) / (np.sqrt(2.0) * s))]
+            sb = blocking_error(series[2501:])
+            # [mean, 10-block error, reference, z(10-block), blocking error, z(blocking)]
+            row[name] = [m, s, float(r), float((m - r) / (np.sqrt(2.0) * s)), sb, float((m - r) / (np.sqrt(2.0) * sb))]
         rows.append(row)
-        print("d_x %.5f  " % D_X[p] + "  ".join("%s %.3e+-%.1e (ref %.3e, z %+.2f)" % (k, *row[k]) for k in
-                                                ("F1x", "F1z", "F2x", "F2z")), flush=True)
+        print("d_x %.5f  " % D_X[p] + "  ".join("%s %.3e+-%.1e (ref %.3e, z %+.2f; blocking +-%.1e, z %+.2f)"
+                                                % (k, *row[k]) for k in ("F1x", "F1z", "F2x", "F2z")), flush=True)
     zs = np.array([row[k][3] for row in rows for k in ("F1x", "F1z", "F2x", "F2z")])
     summary = {"points": len(rows), "wall_s": wall, "max_abs_z": float(np.max(np.abs(zs))),
                "rms_z": float(np.sqrt(np.mean(zs ** 2))), "rows": rows}
