@@ -294,3 +294,16 @@ def test_validation_fixtures_match_the_scan_tables():
     # the field gives Mach 1 at every collision frequency (v_fl = e E / (m_i w_c))
     assert np.allclose(np.array(s.E_FIELD) / np.array(s.W_C), s.E_FIELD[0] / s.W_C[0], rtol=1e-6)
     assert np.all(pt[:, 1] < 0) and np.all(np.abs(pt[:, 3]) > 0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before
+    touching a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "--gpus 2 is not supported" in r.stderr
