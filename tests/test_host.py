@@ -307,3 +307,27 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "--gpus 2 is not supported" in r.stderr
+
+
+def test_field_driven_vz_density_is_exponentially_modified_gaussian():
+    """The field-driven v_z density (opendust.py:738-754) is the
+    exponentially modified Gaussian of a drift v_fl with thermal spread v_T
+    (an exponential of mean v_fl convolved with N(0, v_T^2)): SciPy's
+    exponnorm with K = v_fl / v_T and scale v_T.  The table's CDF knots are
+    that CDF renormalised on [v_min, v_max]."""
+    from scipy import stats
+
+    for v_T, v_fl in ((0.2457, 8 * 0.2457), (1.0, 0.8), (0.3, 2.5)):
+        K = v_fl / v_T
+        f = host._density_field_driven(v_fl, v_T)
+        lo, hi = -4 * v_T, 6 * abs(v_fl) + 5 * v_T
+        v = np.linspace(lo, hi, 301)
+        ref = stats.exponnorm.pdf(v, K, scale=v_T)
+        assert np.allclose(f(v), ref, rtol=1e-9, atol=1e-12 * ref.max())
+        t = host.vz_table(v_fl, v_T)
+        c = lambda x: stats.exponnorm.cdf(x, K, scale=v_T)
+        xi = (c(t.grid_v) - c(lo)) / (c(hi) - c(lo))
+        assert np.max(np.abs(t.grid_xi - xi)) <= 1e-8
+        # the top face's flux weights the same density by v (opendust.py:673-689)
+        flux = host._flux_field_driven(v_fl, v_T)
+        assert np.allclose(flux(v), v * ref, rtol=1e-9, atol=1e-12 * np.max(np.abs(v * ref)))
