@@ -1044,8 +1044,21 @@ __global__ __launch_bounds__(RowShape<T>::TB) void k_rows(const DevParams* __res
       const int jb = P->ion_off, je = P->ion_off + P->n_pad;
       for (int j0 = jb + (int)threadIdx.x; j0 < je; j0 += ROW_TB * TJ) {
         const int j1 = min(j0 + ROW_TB * TJ, je);
-#pragma unroll 4
-        for (int j = j0; j < j1; j += ROW_TB) {
+        // four sources in flight per thread: their loads are issued together,
+        // so one memory latency is spread over 4 x ROW_G pair evaluations
+        int j = j0;
+        for (; j + 3 * ROW_TB < j1; j += 4 * ROW_TB) {
+          const PV p0 = pj[j], p1 = pj[j + ROW_TB], p2 = pj[j + 2 * ROW_TB], p3 = pj[j + 3 * ROW_TB];
+#pragma unroll
+          for (int k = 0; k < ROW_G; ++k) law(own[k], p0, ax[k], ay[k], az[k]);
+#pragma unroll
+          for (int k = 0; k < ROW_G; ++k) law(own[k], p1, ax[k], ay[k], az[k]);
+#pragma unroll
+          for (int k = 0; k < ROW_G; ++k) law(own[k], p2, ax[k], ay[k], az[k]);
+#pragma unroll
+          for (int k = 0; k < ROW_G; ++k) law(own[k], p3, ax[k], ay[k], az[k]);
+        }
+        for (; j < j1; j += ROW_TB) {
           const PV p = pj[j];
 #pragma unroll
           for (int k = 0; k < ROW_G; ++k) law(own[k], p, ax[k], ay[k], az[k]);
