@@ -58,6 +58,13 @@ class ResidentPlasma:
     def close(self):
         self.engine.close()
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def solve(self, dustParticles, n=None):
         """Run n plasma steps (default: the template's n) around `dustParticles`
         (DustParticle objects, SI, as many as the template has).  Returns an

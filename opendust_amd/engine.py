@@ -319,6 +319,13 @@ class Engine:
             self.lib.od_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def __del__(self):
         try:
             self.close()
@@ -454,6 +461,13 @@ class Batch:
         if self._b:
             self.lib.od_batch_destroy(self._b)
             self._b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def __del__(self):
         try:

@@ -97,3 +97,18 @@ def test_engine_host_sampler_matches_oracle(eng):
 
 def test_build_info(eng):
     assert b"gfx950" in eng.load_library().od_build_info()
+
+
+def test_wrappers_are_context_managers(eng):
+    """Engine, Batch and ResidentPlasma release their device state on leaving
+    a with-block (close() is idempotent)."""
+    from opendust_amd import dynamics
+
+    for cls in (eng.Engine, eng.Batch, dynamics.ResidentPlasma):
+        assert hasattr(cls, "__enter__") and hasattr(cls, "__exit__") and hasattr(cls, "close")
+    e = eng.Engine.__new__(eng.Engine)
+    e.lib = eng.load_library()
+    e._ctx = ctypes.c_void_p()
+    with e as same:
+        assert same is e
+    e.close()
