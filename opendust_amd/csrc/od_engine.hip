@@ -213,1674 +213,10 @@ __global__ __launch_bounds__(256) void k_load_pos(const double* __restrict__ X, 
   pj[i] = p;
 }
 
-// ---------------------------------------------------------------------------
-// all-pairs ion-ion force, FP32 with tile-compensated sums.
-// Block = 256 threads, IB i-ions per thread (stride 256).  j runs over one
-// chunk of the padded source array; source positions are wave-uniform, so they
-// arrive through scalar loads and feed the VALU as SGPR operands.  Blocks are
-// ordered chunk-fastest, so consecutive blocks (which the dispatcher spreads
-// over the 8 XCDs) stream different chunks and each XCD's L2 holds one chunk.
-// out: [n_chunks][3][n_pad] partial sums already scaled to kJ/mol/nm.
-// ---------------------------------------------------------------------------
-constexpr int TJ = 64;  // Kahan flush period (sources)
-
-template <int IB>
-__global__ __launch_bounds__(256) void k_pair32(const float4* __restrict__ pi, const float4* __restrict__ pj,
-                                                int n_pad, int n_chunks, int chunk_len, float a, float linv,
-                                                double scale, double* __restrict__ out) {
-  const int chunk = blockIdx.x % n_chunks;
-  const int iblk = blockIdx.x / n_chunks;
-  const int ibase = iblk * (256 * IB) + threadIdx.x;
-  float xi[IB], yi[IB], zi[IB];
-  float ax[IB], ay[IB], az[IB], sx[IB], sy[IB], sz[IB], cx[IB], cy[IB], cz[IB];
-#pragma unroll
-  for (int k = 0; k < IB; ++k) {
-    const float4 p = pi[ibase + k * 256];
-    xi[k] = p.x;
-    yi[k] = p.y;
-    zi[k] = p.z;
-    sx[k] = sy[k] = sz[k] = 0.f;
-    cx[k] = cy[k] = cz[k] = 0.f;
-  }
-  const int j0 = chunk * chunk_len;
-  const int j1 = min(j0 + chunk_len, n_pad);
-  for (int jt = j0; jt < j1; jt += TJ) {
-#pragma unroll
-    for (int k = 0; k < IB; ++k) ax[k] = ay[k] = az[k] = 0.f;
-#pragma unroll 8
-    for (int jj = 0; jj < TJ; ++jj) {
-      const float4 p = pj[jt + jj];
-#pragma unroll
-      for (int k = 0; k < IB; ++k) pair32(xi[k], yi[k], zi[k], p.x, p.y, p.z, a, linv, ax[k], ay[k], az[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < IB; ++k) {
-      kahan_add(sx[k], cx[k], ax[k]);
-      kahan_add(sy[k], cy[k], ay[k]);
-      kahan_add(sz[k], cz[k], az[k]);
-    }
-  }
-  double* o = out + (size_t)chunk * 3 * n_pad;
-#pragma unroll
-  for (int k = 0; k < IB; ++k) {
-    const int i = ibase + k * 256;
-    o[i] = ((double)sx[k] - (double)cx[k]) * scale;
-    o[n_pad + i] = ((double)sy[k] - (double)cy[k]) * scale;
-    o[2 * n_pad + i] = ((double)sz[k] - (double)cz[k]) * scale;
-  }
-}
-
-// all-pairs ion-ion force, plain FP64 (lambda units)
-template <int IB>
-__global__ __launch_bounds__(256) void k_pair64(const double4* __restrict__ pi, const double4* __restrict__ pj,
-                                                int n_pad, int n_chunks, int chunk_len, double rho0, double scale,
-                                                double* __restrict__ out) {
-  const int chunk = blockIdx.x % n_chunks;
-  const int iblk = blockIdx.x / n_chunks;
-  const int ibase = iblk * (256 * IB) + threadIdx.x;
-  double xi[IB], yi[IB], zi[IB], ax[IB], ay[IB], az[IB];
-#pragma unroll
-  for (int k = 0; k < IB; ++k) {
-    const double4 p = pi[ibase + k * 256];
-    xi[k] = p.x;
-    yi[k] = p.y;
-    zi[k] = p.z;
-    ax[k] = ay[k] = az[k] = 0.0;
-  }
-  const int j0 = chunk * chunk_len;
-  const int j1 = min(j0 + chunk_len, n_pad);
-#pragma unroll 4
-  for (int j = j0; j < j1; ++j) {
-    const double4 p = pj[j];
-#pragma unroll
-    for (int k = 0; k < IB; ++k) pair64(xi[k], yi[k], zi[k], p.x, p.y, p.z, rho0, ax[k], ay[k], az[k]);
-  }
-  double* o = out + (size_t)chunk * 3 * n_pad;
-#pragma unroll
-  for (int k = 0; k < IB; ++k) {
-    const int i = ibase + k * 256;
-    o[i] = ax[k] * scale;
-    o[n_pad + i] = ay[k] * scale;
-    o[2 * n_pad + i] = az[k] * scale;
-  }
-}
-
-// F = sum over chunks (fixed order)
-__global__ __launch_bounds__(256) void k_chunk_reduce(const double* __restrict__ part, int n_chunks, int n3,
-                                                      double* __restrict__ F) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n3) return;
-  double s = part[i];
-  for (int c = 1; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
-  F[i] = s;
-}
-
-// ---------------------------------------------------------------------------
-// Symmetric all-pairs ion-ion force, FP32 (every unordered pair once).
-//
-// Ions are cut into blocks of SYM_TB.  A workgroup owns one target block I
-// and `ch` consecutive source blocks J > I (upper triangle); pairs inside a
-// block are done by k_pair32_diag.  Inside a workgroup (SYM_TB / (64 MT)
-// waves) each wave keeps MT target sub-blocks of 64 ions in registers (MT = 8
-// by default; 4 and 16 are autotuner variants) and streams 64-ion source
-// sub-blocks.  Within a 16-lane row, step s pairs lane c's targets with the
-// source of lane (c - s) mod 16: the source position is read through a DPP
-// row-rotate folded into the subtraction, and the source's reaction
-// accumulator is rotated by one lane per step, so after 16 steps it is home.
-// Rows exchange sources (xor 16 / xor 32) between the four phases, which
-// covers all 64 x 64 pairs.  The waves' reactions on a source sub-block
-// are summed in a fixed order through LDS and stored as one partial per
-// (workgroup, source); target sums are Kahan-compensated across sub-blocks.
-// k_sym_reduce adds diagonal, target and source partials in a fixed order.
-// 16 + 3 VALU per unordered pair (3 of them transcendental) plus one
-// 3-register rotate per MT pairs.
-// ---------------------------------------------------------------------------
-constexpr int SYM_TB = 1024;  // ions per block of the triangular schedule
-
-template <int S>
-__device__ __forceinline__ float dpp_ror(float v) {
-  if constexpr (S == 0) {
-    return v;
-  } else {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v),
-                                                                 __builtin_bit_cast(int, v), 0x120 + S, 0xF,
-                                                                 0xF, true));
-  }
-}
-
-template <int MT>
-struct SymT {
-  float x[MT], y[MT], z[MT];
-  float ax[MT], ay[MT], az[MT];
-};
-
-// pair-law parameters of the symmetric kernels (FP32, units r_D_e / log2 e)
-struct SymLaw {
-  float a, linv;
-  float rc, rs, invw;  // truncation (cell-list kernel)
-};
-
-// Empty asm with a read-write register operand: the value must exist at this
-// point, so the optimiser can neither hoist a step's source reads above it nor
-// sink an accumulator chain below it (either blows up register pressure).
-__device__ __forceinline__ void pin(float& v) { asm volatile("" : "+v"(v)); }
-
-// d = (source rotated by S lanes within each row of 16) - target, for four of
-// the wave's target sub-blocks: the rotation rides on the subtraction as a DPP
-// modifier.  s_nop 1 keeps the two wait states a VALU write needs before a
-// DPP read of the same register.
-template <int S>
-__device__ __forceinline__ void sym_diffs4(const float* tx, const float* ty, const float* tz, float xj, float yj,
-                                           float zj, float* dx, float* dy, float* dz) {
-  if constexpr (S == 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      dx[k] = xj - tx[k];
-      dy[k] = yj - ty[k];
-      dz[k] = zj - tz[k];
-    }
-  } else {
-    asm("s_nop 1\n\t"
-        "v_sub_f32_dpp %0, %12, %15 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %1, %13, %16 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %2, %14, %17 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %3, %12, %18 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %4, %13, %19 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %5, %14, %20 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %6, %12, %21 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %7, %13, %22 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %8, %14, %23 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %9, %12, %25 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %10, %13, %26 row_ror:%c24 row_mask:0xf bank_mask:0xf\n\t"
-        "v_sub_f32_dpp %11, %14, %27 row_ror:%c24 row_mask:0xf bank_mask:0xf"
-        : "=&v"(dx[0]), "=&v"(dy[0]), "=&v"(dz[0]), "=&v"(dx[1]), "=&v"(dy[1]), "=&v"(dz[1]), "=&v"(dx[2]),
-          "=&v"(dy[2]), "=&v"(dz[2]), "=&v"(dx[3]), "=&v"(dy[3]), "=&v"(dz[3])
-        : "v"(xj), "v"(yj), "v"(zj), "v"(tx[0]), "v"(ty[0]), "v"(tz[0]), "v"(tx[1]), "v"(ty[1]),
-          "v"(tz[1]), "v"(tx[2]), "v"(ty[2]), "v"(tz[2]), "i"(S), "v"(tx[3]), "v"(ty[3]), "v"(tz[3]));
-  }
-}
-
-template <int S, int MT>
-__device__ __forceinline__ void sym_diffs(const SymT<MT>& T, float xj, float yj, float zj, float* dx, float* dy,
-                                          float* dz) {
-  static_assert(MT % 4 == 0, "targets per wave in groups of four");
-#pragma unroll
-  for (int g = 0; g < MT; g += 4)
-    sym_diffs4<S>(T.x + g, T.y + g, T.z + g, xj, yj, zj, dx + g, dy + g, dz + g);
-}
-
-// CUT: 0 all pairs, 1 hard truncation at rc, 2 OpenMM switch from rs to rc
-template <int S, int MT, int CUT>
-__device__ __forceinline__ void sym_step(SymT<MT>& T, float xj, float yj, float zj, float& bx, float& by,
-                                         float& bz, const SymLaw& L) {
-  float DX[MT], DY[MT], DZ[MT];
-  sym_diffs<S>(T, xj, yj, zj, DX, DY, DZ);
-#pragma unroll
-  for (int k = 0; k < MT; ++k) {
-    const float dx = DX[k];
-    const float dy = DY[k];
-    const float dz = DZ[k];
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32)));
-    const float r = __builtin_amdgcn_sqrtf(r2);
-    const float e = __builtin_amdgcn_exp2f(-r);
-    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(L.a, r, r2));
-    const float t = __builtin_fmaf(r, w, L.linv);
-    float h = e * w * t;
-    if constexpr (CUT == 2) {
-      const float x = __builtin_fminf(__builtin_fmaxf((r - L.rs) * L.invw, 0.0f), 1.0f);
-      const float x2 = x * x;
-      const float Sx = __builtin_fmaf(x2 * x, __builtin_fmaf(x, __builtin_fmaf(x, -6.0f, 15.0f), -10.0f), 1.0f);
-      const float omx = 1.0f - x;
-      const float dS = -30.0f * x2 * omx * omx * L.invw;
-      h = __builtin_fmaf(h, Sx, -(e * w * dS));
-    }
-    if constexpr (CUT >= 1) h = (r < L.rc) ? h : 0.0f;
-    T.ax[k] = __builtin_fmaf(h, dx, T.ax[k]);
-    T.ay[k] = __builtin_fmaf(h, dy, T.ay[k]);
-    T.az[k] = __builtin_fmaf(h, dz, T.az[k]);
-    bx = __builtin_fmaf(-h, dx, bx);
-    by = __builtin_fmaf(-h, dy, by);
-    bz = __builtin_fmaf(-h, dz, bz);
-  }
-  bx = dpp_ror<1>(bx);
-  by = dpp_ror<1>(by);
-  bz = dpp_ror<1>(bz);
-  pin(bx);
-  pin(by);
-  pin(bz);
-#pragma unroll
-  for (int k = 0; k < MT; ++k) {
-    pin(T.ax[k]);
-    pin(T.ay[k]);
-    pin(T.az[k]);
-  }
-}
-
-__device__ __forceinline__ void sym_xchg(int mask, float& xj, float& yj, float& zj, float& bx, float& by,
-                                         float& bz) {
-  xj = __shfl_xor(xj, mask, 64);
-  yj = __shfl_xor(yj, mask, 64);
-  zj = __shfl_xor(zj, mask, 64);
-  bx = __shfl_xor(bx, mask, 64);
-  by = __shfl_xor(by, mask, 64);
-  bz = __shfl_xor(bz, mask, 64);
-}
-
-template <int MT, int CUT, int... S>
-__device__ __forceinline__ void sym_phase(std::integer_sequence<int, S...>, SymT<MT>& T, float xj, float yj,
-                                          float zj, float& bx, float& by, float& bz, const SymLaw& L) {
-  (sym_step<S, MT, CUT>(T, xj, yj, zj, bx, by, bz, L), ...);
-}
-
-// The 64 x 64 pairs of the wave's source sub-block (xj, yj, zj, one per lane)
-// with its MT target sub-blocks: four phases of 16 rotation steps, rows
-// exchanging sources between phases.  On return (bx, by, bz) hold the
-// reaction on the lane's own source and T.a* the targets' accumulators.
-template <int MT, int CUT>
-__device__ __forceinline__ void sym_tile(SymT<MT>& T, float xj, float yj, float zj, float& bx, float& by, float& bz,
-                                         const SymLaw& L) {
-  const auto seq = std::make_integer_sequence<int, 16>{};
-  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
-  sym_xchg(16, xj, yj, zj, bx, by, bz);
-  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
-  sym_xchg(32, xj, yj, zj, bx, by, bz);
-  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
-  sym_xchg(16, xj, yj, zj, bx, by, bz);
-  sym_phase<MT, CUT>(seq, T, xj, yj, zj, bx, by, bz, L);
-  bx = __shfl_xor(bx, 32, 64);
-  by = __shfl_xor(by, 32, 64);
-  bz = __shfl_xor(bz, 32, 64);
-}
-
-// Work-group (I, k) of the launch-order table wg_tab covers target block I and
-// source blocks I+1+k*ch ..; its partials live in slot wg_base[I] + k, so the
-// launch order (longest first) does not change any sum.
-// tp: [n_wg][3][SYM_TB] FP64 target partials; sp: [n_wg][3][ch * SYM_TB] FP32 source partials.
-// Batched simulations: blk_end[I] (first block past I's simulation) and
-// blk_a[I] (its a) per block; null for one simulation (n_blocks, a).
-// Tile shape MT (64-target sub-blocks per wave; the work-group has
-// SYM_TB / (64 MT) waves) and WPE (waves per SIMD the register allocation
-// must allow) are template parameters; the autotuner picks among the
-// instantiated variants (sym_variants).  The next source sub-block's
-// positions are loaded while the current one is processed.
-template <int MT, int WPE>
-__global__ __launch_bounds__(SYM_TB / MT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pair32_sym(
-    const float4* __restrict__ pos, int n_blocks, int ch, const int2* __restrict__ wg_tab,
-    const int* __restrict__ wg_base, float a, float linv, const int* __restrict__ blk_end,
-    const float* __restrict__ blk_a, double* __restrict__ tp, float* __restrict__ sp) {
-  constexpr int NW = SYM_TB / (64 * MT);
-  __shared__ double red[NW][3][64];
-  const int2 ik = wg_tab[blockIdx.x];
-  const int I = ik.x;
-  const size_t slot = (size_t)wg_base[I] + ik.y;
-  const int J0 = I + 1 + ik.y * ch;
-  const int J1 = min(J0 + ch, blk_end ? blk_end[I] : n_blocks);
-  if (blk_a) a = blk_a[I];
-  const int nsub = (J1 - J0) * (SYM_TB / 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t sp_len = (size_t)ch * SYM_TB;
-  SymT<MT> T;
-  float sx[MT], sy[MT], sz[MT], cx[MT], cy[MT], cz[MT];
-#pragma unroll
-  for (int k = 0; k < MT; ++k) {
-    const float4 p = pos[(size_t)I * SYM_TB + wave * (64 * MT) + k * 64 + lane];
-    T.x[k] = p.x;
-    T.y[k] = p.y;
-    T.z[k] = p.z;
-    sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
-  }
-  const float4* src = pos + (size_t)J0 * SYM_TB + lane;
-  float* spw = sp + slot * 3 * sp_len;
-  const SymLaw L{a, linv, 0.f, 0.f, 0.f};
-  float4 p = src[0];
-  for (int b = 0; b < nsub; ++b) {
-    const float4 pn = src[(size_t)min(b + 1, nsub - 1) * 64];
-    float bx = 0.f, by = 0.f, bz = 0.f;
-#pragma unroll
-    for (int k = 0; k < MT; ++k) T.ax[k] = T.ay[k] = T.az[k] = 0.f;
-    sym_tile<MT, 0>(T, p.x, p.y, p.z, bx, by, bz, L);
-#pragma unroll
-    for (int k = 0; k < MT; ++k) {
-      kahan_add(sx[k], cx[k], T.ax[k]);
-      kahan_add(sy[k], cy[k], T.ay[k]);
-      kahan_add(sz[k], cz[k], T.az[k]);
-    }
-    red[wave][0][lane] = bx;
-    red[wave][1][lane] = by;
-    red[wave][2][lane] = bz;
-    __syncthreads();
-    for (int e = threadIdx.x; e < 192; e += 64 * NW) {
-      const int c = e >> 6, l = e & 63;
-      double v = red[0][c][l];
-#pragma unroll
-      for (int w = 1; w < NW; ++w) v += red[w][c][l];
-      spw[c * sp_len + (size_t)b * 64 + l] = (float)v;
-    }
-    __syncthreads();
-    p = pn;
-  }
-  double* tpw = tp + slot * 3 * SYM_TB;
-#pragma unroll
-  for (int k = 0; k < MT; ++k) {
-    const int t = wave * (64 * MT) + k * 64 + lane;
-    tpw[t] = (double)sx[k] - (double)cx[k];
-    tpw[SYM_TB + t] = (double)sy[k] - (double)cy[k];
-    tpw[2 * SYM_TB + t] = (double)sz[k] - (double)cz[k];
-  }
-}
-
-using SymKernel = void (*)(const float4*, int, int, const int2*, const int*, float, float, const int*, const float*,
-                           double*, float*);
-// Variants of the symmetric kernel the autotuner chooses from (od_config.sym_tile
-// - 1): tile shape MT and the waves per SIMD the register budget must allow.
-// As compiled for gfx950: <8,3> 145 VGPRs (3 waves per SIMD), <8,4> 128 (4, a
-// few registers spilled), <4,5> 84 (5), <16,2> 256 (2, a few spilled), <16,1>
-// 264 (1).
-struct SymVariant {
-  SymKernel fn;
-  int mt, wpe;
-};
-static const SymVariant sym_variants[] = {
-    {k_pair32_sym<8, 3>, 8, 3},   {k_pair32_sym<8, 4>, 8, 4},   {k_pair32_sym<4, 5>, 4, 5},
-    {k_pair32_sym<16, 2>, 16, 2}, {k_pair32_sym<16, 1>, 16, 1},
-};
-constexpr int SYM_N_VARIANTS = (int)(sizeof(sym_variants) / sizeof(sym_variants[0]));
-
-// pairs inside one block (ordered, sources via scalar loads); dg: [3][n_pad]
-__global__ __launch_bounds__(256) void k_pair32_diag(const float4* __restrict__ pos, int n_pad, float a,
-                                                     float linv, const float* __restrict__ blk_a,
-                                                     double* __restrict__ dg) {
-  const int X = blockIdx.x / (SYM_TB / 256), q = blockIdx.x % (SYM_TB / 256);
-  if (blk_a) a = blk_a[X];
-  const int i = X * SYM_TB + q * 256 + threadIdx.x;
-  const float4 pi = pos[i];
-  float ax = 0.f, ay = 0.f, az = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-  const float4* src = pos + (size_t)X * SYM_TB;
-  for (int jt = 0; jt < SYM_TB; jt += TJ) {
-    ax = ay = az = 0.f;
-#pragma unroll 8
-    for (int jj = 0; jj < TJ; ++jj) {
-      const float4 p = src[jt + jj];
-      pair32(pi.x, pi.y, pi.z, p.x, p.y, p.z, a, linv, ax, ay, az);
-    }
-    kahan_add(sx, cx, ax);
-    kahan_add(sy, cy, ay);
-    kahan_add(sz, cz, az);
-  }
-  dg[i] = (double)sx - (double)cx;
-  dg[n_pad + i] = (double)sy - (double)cy;
-  dg[2 * n_pad + i] = (double)sz - (double)cz;
-}
-
-// F = scale * (diagonal + target partials + source partials), fixed order.
-// Batched simulations: blk_begin[X] (first block of X's simulation) and
-// blk_scale[X] per block; null for one simulation (0, scale).
-__global__ __launch_bounds__(256) void k_sym_reduce(const double* __restrict__ dg, const double* __restrict__ tp,
-                                                    const float* __restrict__ sp, const int* __restrict__ wg_base,
-                                                    int n_blocks, int ch, int n_pad, double scale,
-                                                    const int* __restrict__ blk_begin,
-                                                    const double* __restrict__ blk_scale, double* __restrict__ F) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_pad) return;
-  const int X = i / SYM_TB, off = i % SYM_TB;
-  const int Jb = blk_begin ? blk_begin[X] : 0;
-  if (blk_scale) scale = blk_scale[X];
-  const size_t sp_len = (size_t)ch * SYM_TB;
-  const int k0 = wg_base[X], k1 = wg_base[X + 1];
-  // the three components walk the partials together (three independent
-  // chains of loads in flight); each component's order is J ascending, then w
-  double s0 = dg[i], s1 = dg[(size_t)n_pad + i], s2 = dg[2 * (size_t)n_pad + i];
-#pragma unroll 4
-  for (int J = Jb; J < X; ++J) {
-    const int d = X - J - 1;
-    const size_t wg = (size_t)wg_base[J] + d / ch;
-    const float* q = sp + wg * 3 * sp_len + (size_t)(d % ch) * SYM_TB + off;
-    s0 += (double)q[0];
-    s1 += (double)q[sp_len];
-    s2 += (double)q[2 * sp_len];
-  }
-#pragma unroll 4
-  for (int w = k0; w < k1; ++w) {
-    const double* t = tp + (size_t)w * 3 * SYM_TB + off;
-    s0 += t[0];
-    s1 += t[SYM_TB];
-    s2 += t[2 * SYM_TB];
-  }
-  F[i] = s0 * scale;
-  F[(size_t)n_pad + i] = s1 * scale;
-  F[2 * (size_t)n_pad + i] = s2 * scale;
-}
-
-// ---------------------------------------------------------------------------
-// step kernels
-// ---------------------------------------------------------------------------
-constexpr uint32_t FLAG_OUT_R = 1u << 16;
-constexpr uint32_t FLAG_OUT_Z = 1u << 17;
-
-// Collisions (fieldDriven, opendust.py:869-873), first half-kick, drift, flags
-// (opendust.py:864-910).  pj gets the drifted position, or the parking spot
-// for flagged ions, which are excluded as sources of the main pair kernel.
-template <class T>
-__global__ __launch_bounds__(256) void k_kick_drift(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
-                                                    double* __restrict__ X, double* __restrict__ V,
-                                                    const double* __restrict__ F, const double* __restrict__ qall,
-                                                    const uint32_t* __restrict__ st,
-                                                    typename Pos<T>::V* __restrict__ pi,
-                                                    typename Pos<T>::V* __restrict__ pj, uint32_t* __restrict__ flags,
-                                                    int* __restrict__ block_cnt, int* __restrict__ err) {
-  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
-  const double* q = qall + P->q_off;
-  const int np = P->stride;
-  const int i = blockIdx.x * 256 + threadIdx.x;  // global index
-  const int li = i - P->ion_off;                 // index within the simulation
-  int flagged = 0;
-  if (li < P->n) {
-    double x = X[i], y = X[np + i], z = X[2 * np + i];
-    double vx = V[i], vy = V[np + i], vz = V[2 * np + i];
-    const uint32_t step = st[0];
-    if (P->distribution == OD_DIST_FIELD_DRIVEN) {
-      const u32x4 r = philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL, P->seed);
-      const bool c0 = r.v[0] < P->thr_coll, c1 = r.v[1] < P->thr_coll, c2 = r.v[2] < P->thr_coll;
-      if (c0 || c1 || c2) {
-        double g0, g1, g2, gd;
-        box_muller(philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL | (1u << 8), P->seed), g0, g1);
-        box_muller(philox4x32_10((uint32_t)li, step, P->sim_id, S_COLL | (2u << 8), P->seed), g2, gd);
-        if (c0) vx = P->v_T * g0;
-        if (c1) vy = P->v_T * g1;
-        if (c2) vz = P->v_T * g2;
-      }
-    }
-    double fx, fy, fz;
-    ext_force(P->ext, q, x, y, z, fx, fy, fz);
-    fx += F[i];
-    fy += F[np + i];
-    fz += F[2 * np + i];
-    vx = vx + P->half_dt * fx / P->m_s;
-    vy = vy + P->half_dt * fy / P->m_s;
-    vz = vz + P->half_dt * fz / P->m_s;
-    x = x + P->dt * vx;
-    y = y + P->dt * vy;
-    z = z + P->dt * vz;
-    uint32_t fl = 0;
-    for (int p = 0; p < P->n_dust; ++p) {
-      const double dx = x - P->ext.dx[p], dy = y - P->ext.dy[p], dz = z - P->ext.dz[p];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      if (-d + P->ext.dr[p] >= 0.0) fl |= 1u << p;
-    }
-    if (-P->R + sqrt(x * x + y * y) >= 0.0) fl |= FLAG_OUT_R;
-    if (-P->half_H + fabs(z) >= 0.0) fl |= FLAG_OUT_Z;
-    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(vx) && isfinite(vy) && isfinite(vz)))
-      atomicOr(err, 1);
-    X[i] = x;
-    X[np + i] = y;
-    X[2 * np + i] = z;
-    V[i] = vx;
-    V[np + i] = vy;
-    V[2 * np + i] = vz;
-    flags[i] = fl;
-    const typename Pos<T>::V p = Pos<T>::make(x, y, z, P->pos_scale);
-    pi[i] = p;
-    pj[i] = fl ? Pos<T>::park() : p;
-    flagged = fl != 0;
-  }
-  const int cnt = __syncthreads_count(flagged);
-  if (threadIdx.x == 0) block_cnt[blockIdx.x] = cnt;
-}
-
-// exclusive scan of the per-block flagged counts (one block)
-__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ block_cnt, int nb, int* __restrict__ block_off,
-                                               int* __restrict__ n_flag) {
-  __shared__ int s[1024];
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
-  int local = 0;
-  for (int b = b0; b < b1; ++b) local += block_cnt[b];
-  s[t] = local;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = (t >= o) ? s[t - o] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  int run = s[t] - local;  // exclusive prefix of this thread's range
-  for (int b = b0; b < b1; ++b) {
-    block_off[b] = run;
-    run += block_cnt[b];
-  }
-  if (t == 1023) n_flag[0] = s[1023];
-}
-
-// Exclusive scan of long int arrays in three launches: tile sums (1024
-// entries per block), a one-block scan of the tile sums (k_scan), and the
-// scan inside each tile plus its offset.  Integer sums: exact in any order.
-__global__ __launch_bounds__(256) void k_tile_sum(const int* __restrict__ in, int n, int* __restrict__ tsum) {
-  __shared__ int red[4];
-  const int base = blockIdx.x * 1024;
-  int v = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = base + r * 256 + (int)threadIdx.x;
-    if (i < n) v += in[i];
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) tsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
-
-__global__ __launch_bounds__(256) void k_tile_scan(const int* __restrict__ in, int n, const int* __restrict__ toff,
-                                                   int* __restrict__ out) {
-  __shared__ int wsum[4];
-  const int i0 = blockIdx.x * 1024 + 4 * (int)threadIdx.x;
-  int v[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : 0;
-  const int t = v[0] + v[1] + v[2] + v[3];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int incl = t;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += u;
-  }
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  int run = toff[blockIdx.x] + incl - t;
-  for (int w = 0; w < wave; ++w) run += wsum[w];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (i0 + k < n) out[i0 + k] = run;
-    run += v[k];
-  }
-}
-
-// ordered list of flagged ions: slot = offset of the block + rank inside it
-template <class T>
-__global__ __launch_bounds__(256) void k_list(const uint32_t* __restrict__ flags, const int* __restrict__ block_cnt,
-                                              const int* __restrict__ block_off, int n,
-                                              const typename Pos<T>::V* __restrict__ pi, int* __restrict__ slot_of,
-                                              int* __restrict__ list_ion, typename Pos<T>::V* __restrict__ list_old) {
-  if (block_cnt[blockIdx.x] == 0) return;
-  __shared__ int wcnt[4];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool f = (i < n) && flags[i] != 0u;
-  const unsigned long long m = __ballot(f);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r_in_wave = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) wcnt[wave] = __popcll(m);
-  __syncthreads();
-  int before = 0;
-  for (int w = 0; w < wave; ++w) before += wcnt[w];
-  if (f) {
-    const int s = block_off[blockIdx.x] + before + r_in_wave;
-    slot_of[i] = s;
-    list_ion[s] = i;
-    list_old[s] = pi[i];
-  }
-}
-
-// Reinjection of one flagged ion (opendust.py:923-975), FP64.
-// face: 1 top, 2 side, 3 bottom.
-__device__ int reinject(const DevParams& P, uint32_t ion, uint32_t step, double& x, double& y, double& z, double& vx,
-                        double& vy, double& vz) {
-  const double two_pi = 2.0 * 3.141592653589793;
-  const uint32_t uf = philox4x32_10(ion, step, P.sim_id, S_FACE, P.seed).v[0];
-  int face;
-  if ((uint64_t)uf < P.thr_top)
-    face = 1;
-  else if ((uint64_t)uf < P.thr_side)
-    face = 2;
-  else
-    face = 3;
-  if (face == 1 || face == 3) {
-    const bool top = face == 1;
-    double g1, g2;
-    box_muller(philox4x32_10(ion, step, P.sim_id, top ? S_TOPV : S_BOTV, P.seed), g1, g2);
-    const u32x4 rz = philox4x32_10(ion, step, P.sim_id, (top ? S_TOPV : S_BOTV) | (1u << 8), P.seed);
-    const double uz = u53(rz.v[0], rz.v[1]);
-    vx = P.v_T * g1;
-    vy = P.v_T * g2;
-    vz = table_eval(top ? P.top : P.bottom, uz);
-    const u32x4 rx = philox4x32_10(ion, step, P.sim_id, top ? S_TOPX : S_BOTX, P.seed);
-    const double ux = u53(rx.v[0], rx.v[1]), uy = u53(rx.v[2], rx.v[3]);
-    const double rr = P.R * sqrt(ux);
-    x = rr * cos(two_pi * uy);
-    y = rr * sin(two_pi * uy);
-    z = top ? -P.half_H : P.half_H;
-  } else {
-    const u32x4 rx = philox4x32_10(ion, step, P.sim_id, S_SIDEX, P.seed);
-    const double uy = u53(rx.v[0], rx.v[1]), uz = u53(rx.v[2], rx.v[3]);
-    x = P.R * cos(two_pi * uy);
-    y = P.R * sin(two_pi * uy);
-    z = uz * P.H - P.half_H;
-    double gy, gz;
-    box_muller(philox4x32_10(ion, step, P.sim_id, S_SIDEV, P.seed), gy, gz);
-    const u32x4 ru = philox4x32_10(ion, step, P.sim_id, S_SIDEV | (1u << 8), P.seed);
-    const double us = u53(ru.v[0], ru.v[1]);
-    const double vs = table_eval(P.side, us);
-    vx = -vs * (x / P.R) + P.v_T * gy * (y / P.R);
-    vy = -vs * (y / P.R) - P.v_T * gy * (x / P.R);
-    if (P.distribution == OD_DIST_FIELD_DRIVEN)
-      vz = table_eval(P.vz, us);
-    else
-      vz = gz * P.v_T + P.v_fl;
-  }
-  return face;
-}
-
-// Second half-kick with the flagged-source correction, absorbed counts and
-// momenta (opendust.py:911-921), reinjection, orbit sums at the final
-// positions (opendust.py:977-1037).  partials: [nb][OD_MAX_DUST][8].
-template <class T>
-__global__ __launch_bounds__(256) void k_kick2(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
-                                               const int* __restrict__ block_off, int nb_all,
-                                               double* __restrict__ X, double* __restrict__ V,
-                                               const double* __restrict__ F, const double* __restrict__ qall,
-                                               const uint32_t* __restrict__ st,
-                                               const typename Pos<T>::V* __restrict__ pi,
-                                               typename Pos<T>::V* __restrict__ pj,
-                                               const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
-                                               const int* __restrict__ slot_of,
-                                               const typename Pos<T>::V* __restrict__ list_old,
-                                               typename Pos<T>::V* __restrict__ list_new, uint8_t* __restrict__ face_out,
-                                               const double* __restrict__ rowD, const double* __restrict__ G,
-                                               double* __restrict__ partials) {
-  __shared__ double lds[4];
-  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
-  const double* q = qall + P->q_off;
-  const int np = P->stride;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int li = i - P->ion_off;
-  const bool valid = li < P->n;
-  const typename Pos<T>::V own = valid ? pi[i] : Pos<T>::park();
-  Acc<T> ax, ay, az;
-  // this simulation's flagged ions occupy list slots [l0, l1); with G the
-  // flagged sources' share was computed in cell order (cutoff mode)
-  const int l0 = block_off[P->blk0];
-  const int l1 = G ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
-  const PlainLaw<T> law(*P);  // the list loop runs in all-pairs mode only
-  for (int s0 = l0; s0 < l1; s0 += TJ) {
-    const int s1 = min(s0 + TJ, l1);
-    for (int s = s0; s < s1; ++s) law(own, list_old[s], ax, ay, az);
-    ax.flush();
-    ay.flush();
-    az.flush();
-  }
-  uint32_t fl = 0;
-  double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0, px = 0, py = 0, pz = 0;
-  if (valid) {
-    fl = flags[i];
-    x = X[i];
-    y = X[np + i];
-    z = X[2 * np + i];
-    vx = V[i];
-    vy = V[np + i];
-    vz = V[2 * np + i];
-    double fx, fy, fz;
-    ext_force(P->ext, q, x, y, z, fx, fy, fz);
-    if (fl & P->dust_mask) {
-      // absorbed: its full row at the drifted state (the main kernel parks flagged ions)
-      const int s = slot_of[i];
-      fx += rowD[3 * s];
-      fy += rowD[3 * s + 1];
-      fz += rowD[3 * s + 2];
-    } else {
-      // flagged ions that only left the domain are reinjected below, whatever this kick gives
-      if (G) {
-        fx += F[i] + G[i];
-        fy += F[np + i] + G[np + i];
-        fz += F[2 * np + i] + G[2 * np + i];
-      } else {
-        fx += F[i] + ax.total() * P->pair_scale;
-        fy += F[np + i] + ay.total() * P->pair_scale;
-        fz += F[2 * np + i] + az.total() * P->pair_scale;
-      }
-    }
-    vx = vx + P->half_dt * fx / P->m_s;
-    vy = vy + P->half_dt * fy / P->m_s;
-    vz = vz + P->half_dt * fz / P->m_s;
-    px = vx;
-    py = vy;
-    pz = vz;
-    int face = 0;
-    if (fl) {
-      face = reinject(*P, (uint32_t)li, st[0], x, y, z, vx, vy, vz);
-      X[i] = x;
-      X[np + i] = y;
-      X[2 * np + i] = z;
-      const typename Pos<T>::V p = Pos<T>::make(x, y, z, P->pos_scale);
-      pj[i] = p;
-      list_new[slot_of[i]] = p;
-    }
-    V[i] = vx;
-    V[np + i] = vy;
-    V[2 * np + i] = vz;
-    face_out[i] = (uint8_t)face;
-  }
-  for (int p = 0; p < P->n_dust; ++p) {
-    const double in = (valid && ((fl >> p) & 1u)) ? 1.0 : 0.0;
-    double ox = 0, oy = 0, oz = 0;
-    if (valid) {
-      const double xp = P->ext.dx[p], yp = P->ext.dy[p], zp = P->ext.dz[p], lam = P->lam;
-      const double dx = x - xp, dy = y - yp, dz = z - zp;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const double e = exp(-r / lam);
-      const double r3 = r * r * r, r2 = r * r, l2 = lam * lam;
-      ox = e * (xp - x) / r3 * l2 + e * (xp - x) / r2 * lam;
-      oy = e * (yp - y) / r3 * l2 + e * (yp - y) / r2 * lam;
-      oz = e * (zp - z) / r3 * l2 + e * (zp - z) / r2 * lam;
-    }
-    const double v[7] = {in, in * px, in * py, in * pz, ox, oy, oz};
-    double* dst = partials + ((size_t)blockIdx.x * OD_MAX_DUST + p) * 8;
-    for (int k = 0; k < 7; ++k) {
-      const double s = block_sum256(v[k], lds);
-      if (threadIdx.x == 0) dst[k] = s;
-    }
-  }
-}
-
-// Full force rows of listed ions against all sources `pj`: the reinjected ions
-// at the post-reinjection state (need_mask = 0), or the absorbed ions at the
-// drifted state (need_mask = dust bits: other listed ions are skipped).
-// A work-group of RowShape<T>::TB threads takes RowShape<T>::G consecutive
-// list slots and streams its simulation's sources once for all of them (each
-// source load feeds G pairs); slots of another simulation in the same group
-// (batched runs) get a pass of their own.  FP32: 4 slots, 16 waves (about
-// 1200 reinjected ions per step at 2^20 give ~300 work-groups, each 16 waves
-// deep for latency hiding); FP64: 2 slots, 8 waves (twice the groups for the
-// heavier pairs).  Per thread: sources strided by TB, Kahan flush every
-// TJ of them; then a fixed-order sum over the work-group.  All-pairs mode only
-// (the cell-list mode has k_rows_cell).
-template <class T>
-struct RowShape;
-template <>
-struct RowShape<float> {
-  static constexpr int G = 4, TB = 1024;
-};
-template <>
-struct RowShape<double> {
-  static constexpr int G = 2, TB = 512;
-};
-
-template <class T>
-__global__ __launch_bounds__(RowShape<T>::TB) void k_rows(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
-                                                 const typename Pos<T>::V* __restrict__ pj,
-                                                 const int* __restrict__ n_flag,
-                                                 const typename Pos<T>::V* __restrict__ list_new,
-                                                 const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
-                                                 uint32_t need_mask, double* __restrict__ rowF) {
-  using PV = typename Pos<T>::V;
-  constexpr int ROW_G = RowShape<T>::G, ROW_TB = RowShape<T>::TB;
-  constexpr int NW = ROW_TB / 64, NV = 3 * ROW_G;
-  __shared__ double lds[NW][NV];
-  const int nf = n_flag[0];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int s0 = blockIdx.x * ROW_G; s0 < nf; s0 += gridDim.x * ROW_G) {
-    uint32_t todo = 0;
-#pragma unroll
-    for (int k = 0; k < ROW_G; ++k) {
-      const int s = s0 + k;
-      if (s < nf && (need_mask == 0u || (flags[list_ion[s]] & need_mask) != 0u)) todo |= 1u << k;
-    }
-    while (todo) {
-      const int sim_blk = list_ion[s0 + __builtin_ctz(todo)] >> 8;
-      const DevParams* P = sim_of_block(Pall, blk_sim, sim_blk);
-      uint32_t mine = 0;
-      PV own[ROW_G];
-#pragma unroll
-      for (int k = 0; k < ROW_G; ++k) {
-        own[k] = Pos<T>::park();
-        if (((todo >> k) & 1u) && sim_of_block(Pall, blk_sim, list_ion[s0 + k] >> 8) == P) {
-          mine |= 1u << k;
-          own[k] = list_new[s0 + k];
-        }
-      }
-      todo &= ~mine;
-      Acc<T> ax[ROW_G], ay[ROW_G], az[ROW_G];
-      const PlainLaw<T> law(*P);
-      const int jb = P->ion_off, je = P->ion_off + P->n_pad;
-      for (int j0 = jb + (int)threadIdx.x; j0 < je; j0 += ROW_TB * TJ) {
-        const int j1 = min(j0 + ROW_TB * TJ, je);
-        // four sources in flight per thread: their loads are issued together,
-        // so one memory latency is spread over 4 x ROW_G pair evaluations
-        int j = j0;
-        for (; j + 3 * ROW_TB < j1; j += 4 * ROW_TB) {
-          const PV p0 = pj[j], p1 = pj[j + ROW_TB], p2 = pj[j + 2 * ROW_TB], p3 = pj[j + 3 * ROW_TB];
-#pragma unroll
-          for (int k = 0; k < ROW_G; ++k) law(own[k], p0, ax[k], ay[k], az[k]);
-#pragma unroll
-          for (int k = 0; k < ROW_G; ++k) law(own[k], p1, ax[k], ay[k], az[k]);
-#pragma unroll
-          for (int k = 0; k < ROW_G; ++k) law(own[k], p2, ax[k], ay[k], az[k]);
-#pragma unroll
-          for (int k = 0; k < ROW_G; ++k) law(own[k], p3, ax[k], ay[k], az[k]);
-        }
-        for (; j < j1; j += ROW_TB) {
-          const PV p = pj[j];
-#pragma unroll
-          for (int k = 0; k < ROW_G; ++k) law(own[k], p, ax[k], ay[k], az[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < ROW_G; ++k) {
-          ax[k].flush();
-          ay[k].flush();
-          az[k].flush();
-        }
-      }
-      double v[NV];
-#pragma unroll
-      for (int k = 0; k < ROW_G; ++k) {
-        v[3 * k + 0] = ax[k].total();
-        v[3 * k + 1] = ay[k].total();
-        v[3 * k + 2] = az[k].total();
-      }
-#pragma unroll
-      for (int m = 0; m < NV; ++m) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v[m] += __shfl_xor(v[m], o, 64);
-      }
-      if (lane == 0) {
-#pragma unroll
-        for (int m = 0; m < NV; ++m) lds[wave][m] = v[m];
-      }
-      __syncthreads();
-      if (threadIdx.x < NV) {
-        const int m = threadIdx.x, k = m / 3;
-        double s = lds[0][m];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) s += lds[w][m];
-        if ((mine >> k) & 1u) rowF[3 * (s0 + k) + m % 3] = s * P->pair_scale;
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Force of the reinjected ions on the kept ions; reinjected ions take their rows.
-template <class T>
-__global__ __launch_bounds__(256) void k_inflow(const DevParams* __restrict__ Pall, const int* __restrict__ blk_sim,
-                                                const int* __restrict__ block_off, int nb_all, double* __restrict__ F,
-                                                const typename Pos<T>::V* __restrict__ pj,
-                                                const uint32_t* __restrict__ flags, const int* __restrict__ n_flag,
-                                                const typename Pos<T>::V* __restrict__ list_new,
-                                                const int* __restrict__ slot_of, const double* __restrict__ rowF,
-                                                int rows_only) {
-  const DevParams* P = sim_of_block(Pall, blk_sim, blockIdx.x);
-  const int np = P->stride;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i - P->ion_off < P->n;
-  const typename Pos<T>::V own = valid ? pj[i] : Pos<T>::park();
-  Acc<T> ax, ay, az;
-  // rows_only: the kept ions' share was added in cell order (cutoff mode)
-  const int l0 = block_off[P->blk0];
-  const int l1 = rows_only ? l0 : ((P->blk1 < nb_all) ? block_off[P->blk1] : n_flag[0]);
-  const PlainLaw<T> law(*P);  // the list loop runs in all-pairs mode only
-  for (int s0 = l0; s0 < l1; s0 += TJ) {
-    const int s1 = min(s0 + TJ, l1);
-    for (int s = s0; s < s1; ++s) law(own, list_new[s], ax, ay, az);
-    ax.flush();
-    ay.flush();
-    az.flush();
-  }
-  if (!valid) return;
-  if (flags[i]) {
-    const int s = slot_of[i];
-    F[i] = rowF[3 * s + 0];
-    F[np + i] = rowF[3 * s + 1];
-    F[2 * np + i] = rowF[3 * s + 2];
-  } else if (!rows_only) {
-    F[i] += ax.total() * P->pair_scale;
-    F[np + i] += ay.total() * P->pair_scale;
-    F[2 * np + i] += az.total() * P->pair_scale;
-  }
-}
-
-// Dust records: FP64 block-partial reduction (fixed order), charge update
-// (opendust.py:1117-1202), dust forces (opendust.py:1204-1384).
-// One block per simulation (grid = number of simulations); records go to
-// rec[(step - base) * n_sims + sim][nd_max].  The step counter is advanced by
-// block 0 when `advance` is set (one simulation), else by k_step_advance.
-__global__ __launch_bounds__(256) void k_dust(const DevParams* __restrict__ Pall, const double* __restrict__ partials,
-                                              double* __restrict__ qall, uint32_t* __restrict__ st,
-                                              od_dust_record* __restrict__ rec, int nd_max, int advance) {
-  __shared__ double lds[4];
-  __shared__ double tot[OD_MAX_DUST][8];
-  const DevParams* P = Pall + blockIdx.x;
-  double* q = qall + P->q_off;
-  const int nd = P->n_dust;
-  for (int p = 0; p < nd; ++p) {
-    for (int k = 0; k < 7; ++k) {
-      double s = 0.0;
-      for (int b = P->blk0 + (int)threadIdx.x; b < P->blk1; b += 256)
-        s += partials[((size_t)b * OD_MAX_DUST + p) * 8 + k];
-      s = block_sum256(s, lds);
-      if (threadIdx.x == 0) tot[p][k] = s;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const uint32_t i = st[0];
-  const uint32_t ri = i - st[1];
-  const double pi_ = 3.141592653589793;
-  double qn[OD_MAX_DUST];
-  for (int p = 0; p < nd; ++p) {
-    const double qp = q[p];
-    qn[p] = qp;
-    const int m = P->dust_method[p];
-    if (i > 0 && (m == OD_CHARGE_CONSISTENT || m == OD_CHARGE_OML)) {
-      const double rp = P->si_dr[p];
-      const double phi = P->si_k * qp / rp;
-      const double gamma_e = P->gamma_e_pref * exp(-P->si_e * phi / P->si_Te / P->si_kB);
-      const double S = rp * rp * 4.0 * pi_;
-      double dq;
-      if (m == OD_CHARGE_CONSISTENT) {
-        dq = tot[p][0] * P->si_q_s + gamma_e * P->si_dt * S * P->si_e;
-      } else {
-        const double chi = P->si_e * phi / P->si_Tn / P->si_kB;
-        const double gamma_i = P->oml_A * ((P->oml_B + chi / P->oml_u2) * P->oml_erf_u + P->oml_tail);
-        dq = gamma_i * P->si_dt * S * fabs(P->si_e) + gamma_e * P->si_dt * S * P->si_e;
-      }
-      qn[p] = qp + dq;
-    }
-  }
-  const double lam = P->si_lam, k = P->si_k, qs = P->si_q_s;
-  const double* al = P->ext.alpha;
-  for (int p = 0; p < nd; ++p) {
-    od_dust_record r;
-    r.q = qn[p];
-    const double xp = P->si_dx[p], yp = P->si_dy[p], zp = P->si_dz[p];
-    double fdx = 0, fdy = 0, fdz = 0;
-    for (int j = 0; j < nd; ++j) {
-      if (j == p) continue;
-      const double dx = xp - P->si_dx[j], dy = yp - P->si_dy[j], dz = zp - P->si_dz[j];
-      const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-      const double r3 = rr * rr * rr;
-      fdx += k * qn[p] * qn[j] * dx / r3;
-      fdy += k * qn[p] * qn[j] * dy / r3;
-      fdz += k * qn[p] * qn[j] * dz / r3;
-    }
-    r.force_dust[0] = fdx;
-    r.force_dust[1] = fdy;
-    r.force_dust[2] = fdz;
-    // in-domain (orbit) part
-    const double cq = qn[p] * k * qs;
-    const double l2 = lam * lam;
-    double fo[3];
-    for (int c = 0; c < 3; ++c) fo[c] = tot[p][4 + c] * cq / l2;
-    // out-of-domain part: analytic -grad of the trap polynomial at the dust
-    const double rr = sqrt(xp * xp + yp * yp);
-    const double z = zp;
-    const double r2 = rr * rr, z2 = z * z, l4 = l2 * l2, l6 = l4 * l2, l8 = l4 * l4;
-    const double r3 = r2 * rr, r5 = r3 * r2, r7 = r5 * r2, r4 = r2 * r2, r6 = r4 * r2;
-    const double z3 = z2 * z, z4 = z2 * z2, z5 = z4 * z, z6 = z4 * z2, z7 = z6 * z;
-    double fR = (-2 * al[1] * rr) / l2;
-    fR += (-2 * al[3] * rr * z2 - 4 * al[4] * r3) / l4;
-    fR += (-4 * al[6] * r3 * z2 - 2 * al[7] * rr * z4 - 6 * al[8] * r5) / l6;
-    fR += (-4 * al[10] * r3 * z4 - 6 * al[11] * r5 * z2 - 2 * al[12] * rr * z6 - 8 * al[13] * r7) / l8;
-    fR *= k * qn[p] * qs / lam;
-    double fH = (-2 * al[2] * z) / l2;
-    fH += (-2 * al[3] * r2 * z - 4 * al[5] * z3) / l4;
-    fH += (-2 * al[6] * r4 * z - 4 * al[7] * r2 * z3 - 6 * al[9] * z5) / l6;
-    fH += (-4 * al[10] * r4 * z3 - 2 * al[11] * r6 * z - 6 * al[12] * r2 * z5 - 8 * al[14] * z7) / l8;
-    fH *= k * qn[p] * qs / lam;
-    double ofx = 0, ofy = 0;
-    if (rr != 0.0) {
-      ofx = fR * xp / rr;
-      ofy = fR * yp / rr;
-    }
-    r.force_orbit[0] = fo[0] + ofx;
-    r.force_orbit[1] = fo[1] + ofy;
-    r.force_orbit[2] = fo[2] + fH;
-    for (int c = 0; c < 3; ++c) {
-      r.force_collect[c] = tot[p][1 + c] * P->collect_scale;
-      r.orbit_sum[c] = tot[p][4 + c];
-      r.momentum_sum[c] = tot[p][1 + c];
-    }
-    r.force_field[0] = 0.0;
-    r.force_field[1] = 0.0;
-    r.force_field[2] = (P->distribution == OD_DIST_FIELD_DRIVEN) ? qn[p] * P->si_E : 0.0;
-    r.absorbed = (int64_t)tot[p][0];
-    rec[((size_t)ri * gridDim.x + blockIdx.x) * nd_max + p] = r;
-  }
-  for (int p = 0; p < nd; ++p) q[p] = qn[p];
-  if (advance) st[0] = i + 1;
-}
-
-__global__ void k_step_advance(uint32_t* __restrict__ st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st[0] = st[0] + 1;
-}
-
-// ---------------------------------------------------------------------------
-// trap background potential (Monte Carlo, opendust.py:499-523)
-// ---------------------------------------------------------------------------
-__host__ __device__ inline void trap_sample(double R, double H, uint64_t seed, int64_t k, double& x, double& y,
-                                            double& z) {
-  const u32x4 a = philox4x32_10((uint32_t)k, (uint32_t)((uint64_t)k >> 32), 0u, S_TRAP, seed);
-  const u32x4 b = philox4x32_10((uint32_t)k, (uint32_t)((uint64_t)k >> 32), 0u, S_TRAP | (1u << 8), seed);
-  const double r = sqrt(u53(a.v[0], a.v[1]));
-  const double theta = u53(a.v[2], a.v[3]);
-  x = R * r * cos(2.0 * 3.141592653589793 * theta);
-  y = R * r * sin(2.0 * 3.141592653589793 * theta);
-  z = -H / 2.0 + (H / 2.0 - (-H / 2.0)) * u53(b.v[0], b.v[1]);
-}
-
-__global__ __launch_bounds__(256) void k_trap_samples(double R, double H, uint64_t seed, int64_t n,
-                                                      double* __restrict__ xs, double* __restrict__ ys,
-                                                      double* __restrict__ zs) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  double x, y, z;
-  trap_sample(R, H, seed, k, x, y, z);
-  xs[k] = x;
-  ys[k] = y;
-  zs[k] = z;
-}
-
-__global__ __launch_bounds__(256) void k_trap_grid(const double* __restrict__ rho, const double* __restrict__ hh,
-                                                   int n_h, const double* __restrict__ xs,
-                                                   const double* __restrict__ ys, const double* __restrict__ zs,
-                                                   int64_t n, int64_t chunk, int n_chunks, double* __restrict__ part) {
-  __shared__ double lds[4];
-  const int gp = blockIdx.x / n_chunks, c = blockIdx.x % n_chunks;
-  const double gR = rho[gp / n_h], gH = hh[gp % n_h];
-  const int64_t k0 = (int64_t)c * chunk, k1 = min(n, k0 + chunk);
-  double s = 0.0;
-  for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
-    const double dx = gR - xs[k], dz = gH - zs[k], y = ys[k];
-    const double d = sqrt(dx * dx + y * y + dz * dz);
-    s += exp(-d) / (d + 0.001);
-  }
-  s = block_sum256(s, lds);
-  if (threadIdx.x == 0) part[(size_t)gp * n_chunks + c] = s;
-}
-
-__global__ __launch_bounds__(256) void k_trap_finish(const double* __restrict__ part, int n_gp, int n_chunks,
-                                                     double n_mc, double* __restrict__ out) {
-  const int gp = blockIdx.x * 256 + threadIdx.x;
-  if (gp >= n_gp) return;
-  double s = 0.0;
-  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)gp * n_chunks + c];
-  out[gp] = s / n_mc;
-}
-
-// ---------------------------------------------------------------------------
-// Cutoff mode: cell list built every step by integer binning and a stable,
-// deterministic LSD radix sort (8-bit digits, two passes for 16-bit cell ids;
-// within a digit, items keep their input order), then a pair kernel over the
-// 5x5x5 cell stencil (cells of cutoff/2).
-// ---------------------------------------------------------------------------
-constexpr int RS_ITEMS = 1024;  // items per radix-sort block (256 threads x 4 rounds)
-
-__device__ __forceinline__ int cell_of(const DevParams& P, double x, double y, double z) {
-  int ix = (int)floor((x + P.R) / P.cell_h);
-  int iy = (int)floor((y + P.R) / P.cell_h);
-  int iz = (int)floor((z + P.half_H) / P.cell_h);
-  ix = min(max(ix, 0), P.nx - 1);
-  iy = min(max(iy, 0), P.ny - 1);
-  iz = min(max(iz, 0), P.nz - 1);
-  return ix + P.nx * (iy + P.ny * iz);
-}
-
-// key = cell of the ion (FP64 position), or n_cells for ions left out (flagged,
-// or padding); value = ion index
-__global__ __launch_bounds__(256) void k_bin(const DevParams* __restrict__ P, const double* __restrict__ X,
-                                             const uint32_t* __restrict__ flags, uint32_t* __restrict__ key,
-                                             uint32_t* __restrict__ val) {
-  const int np = P->n_pad;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= np) return;
-  uint32_t k = (uint32_t)P->n_cells;
-  if (i < P->n && (flags == nullptr || flags[i] == 0u)) k = (uint32_t)cell_of(*P, X[i], X[np + i], X[2 * np + i]);
-  key[i] = k;
-  val[i] = (uint32_t)i;
-}
-
-// per-block digit histogram, digit-major: hist[d * nblk + b]
-__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ key, int n, int shift, int nblk,
-                                                 int* __restrict__ hist) {
-  __shared__ int h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int base = blockIdx.x * RS_ITEMS;
-  for (int r = 0; r < RS_ITEMS / 256; ++r) {
-    const int i = base + r * 256 + threadIdx.x;
-    if (i < n) atomicAdd(&h[(key[i] >> shift) & 255u], 1);
-  }
-  __syncthreads();
-  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// stable scatter: rank = (digit base of this block) + (same-digit items earlier in the block)
-__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ key_in,
-                                                    const uint32_t* __restrict__ val_in, int n, int shift, int nblk,
-                                                    const int* __restrict__ off, uint32_t* __restrict__ key_out,
-                                                    uint32_t* __restrict__ val_out) {
-  __shared__ int run[256];
-  __shared__ int wc[4][256];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  run[t] = off[t * nblk + blockIdx.x];
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  const int base = blockIdx.x * RS_ITEMS;
-  for (int r = 0; r < RS_ITEMS / 256; ++r) {
-    for (int w = 0; w < 4; ++w) wc[w][t] = 0;
-    __syncthreads();
-    const int i = base + r * 256 + t;
-    const bool valid = i < n;
-    const uint32_t k = valid ? key_in[i] : 0u;
-    const uint32_t d = (k >> shift) & 255u;
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const unsigned long long m = __ballot(valid && ((d >> b) & 1u));
-      peers &= ((d >> b) & 1u) ? m : ~m;
-    }
-    const int rank = __popcll(peers & lt);
-    if (valid && rank == 0) wc[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int pos = run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wc[w][d];
-      key_out[pos] = k;
-      val_out[pos] = val_in[i];
-    }
-    __syncthreads();
-    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-    __syncthreads();
-  }
-}
-
-// cell_start[c] = first position of the sorted keys with key >= c, c <= n_cells
-// (keys of ions left out are n_cells): position i fills the cells between
-// the previous key and its own.
-__global__ __launch_bounds__(256) void k_cell_starts(const uint32_t* __restrict__ skey, int n, int n_cells,
-                                                     int* __restrict__ cell_start) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i > n) return;
-  const int prev = (i == 0) ? -1 : min((int)skey[i - 1], n_cells);
-  const int cur = (i == n) ? n_cells : min((int)skey[i], n_cells);
-  for (int cc = prev + 1; cc <= cur; ++cc) cell_start[cc] = i;
-}
-
-__global__ __launch_bounds__(256) void k_cell_chunks(const int* __restrict__ cell_start, int n_cells,
-                                                     int* __restrict__ chunks) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < n_cells) chunks[c] = (cell_start[c + 1] - cell_start[c] + 63) / 64;
-}
-
-// work items (cell, 64-target chunk), packed as cell | chunk << 16
-__global__ __launch_bounds__(256) void k_cell_items(const int* __restrict__ chunks, const int* __restrict__ item_off,
-                                                    int n_cells, int* __restrict__ items) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= n_cells) return;
-  const int o = item_off[c];
-  for (int k = 0; k < chunks[c]; ++k) items[o + k] = c | (k << 16);
-}
-
-// sorted source positions (kept ions) and their ion ids
-template <class T>
-__global__ __launch_bounds__(256) void k_cell_gather(const uint32_t* __restrict__ skey,
-                                                     const uint32_t* __restrict__ sval, int n_pad, int n_cells,
-                                                     const typename Pos<T>::V* __restrict__ pos,
-                                                     typename Pos<T>::V* __restrict__ spos) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= n_pad) return;
-  spos[p] = ((int)skey[p] < n_cells) ? pos[sval[p]] : Pos<T>::park();
-}
-
-// Truncated pair law with its parameters held in registers (SW: OpenMM switch).
-template <class T, int SW>
-struct CutLaw;
-template <int SW>
-struct CutLaw<float, SW> {
-  float a, linv, rc, rs, invw;
-  __device__ explicit CutLaw(const DevParams& P)
-      : a(P.a32), linv(P.linv32), rc(P.rc32), rs(P.rs32), invw(P.invw32) {}
-  __device__ __forceinline__ void operator()(const float4& i, const float4& j, Acc<float>& ax, Acc<float>& ay,
-                                             Acc<float>& az) const {
-    pair32_cut<SW != 0>(i.x, i.y, i.z, j.x, j.y, j.z, a, linv, rc, rs, invw, ax.a, ay.a, az.a);
-  }
-};
-template <int SW>
-struct CutLaw<double, SW> {
-  double rho0, rc, rs, invw;
-  __device__ explicit CutLaw(const DevParams& P) : rho0(P.rho0), rc(P.rc64), rs(P.rs64), invw(P.invw64) {}
-  __device__ __forceinline__ void operator()(const double4& i, const double4& j, Acc<double>& ax,
-                                             Acc<double>& ay, Acc<double>& az) const {
-    pair64_cut(i.x, i.y, i.z, j.x, j.y, j.z, rho0, SW != 0, rc, rs, invw, ax.a, ay.a, az.a);
-  }
-};
-
-// Forces on one wave's 64 targets (one cell, one chunk) from the 5x5x5 stencil.
-// The item index is made wave-uniform explicitly, so cell bounds and source
-// positions arrive through scalar loads and feed the VALU as SGPR operands;
-// the law's parameters live in registers; pairs beyond the cutoff are masked.
-template <class T, int SW>
-__global__ __launch_bounds__(256) void k_pair_cell(const DevParams* __restrict__ P,
-                                                   const typename Pos<T>::V* __restrict__ spos,
-                                                   const uint32_t* __restrict__ sval,
-                                                   const int* __restrict__ cell_start, const int* __restrict__ items,
-                                                   const int* __restrict__ n_items, double* __restrict__ F) {
-  const CutLaw<T, SW> law(*P);
-  const int lane = threadIdx.x & 63;
-  const int np = P->n_pad;
-  const int nx = P->nx, ny = P->ny, nz = P->nz;
-  const double scale = P->pair_scale;
-  const int nit = n_items[0];
-  const int nw = gridDim.x * 4;
-  for (int it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)); it < nit; it += nw) {
-    const int packed = items[it];
-    const int c = packed & 0xFFFF, k = packed >> 16;
-    const int t0 = cell_start[c] + k * 64;
-    const int tp = t0 + lane;
-    const bool valid = tp < cell_start[c + 1];
-    const typename Pos<T>::V own = valid ? spos[tp] : Pos<T>::park();
-    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-    Acc<T> ax, ay, az;
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z) {
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
-        const int row = nx * (y + ny * z);
-        const int j0 = cell_start[row + max(cx - 2, 0)];
-        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
-        for (int jt = j0; jt < j1; jt += TJ) {
-          const int je = min(jt + TJ, j1);
-          int j = jt;
-          for (; j + 4 <= je; j += 4) {
-            const typename Pos<T>::V p0 = spos[j], p1 = spos[j + 1], p2 = spos[j + 2], p3 = spos[j + 3];
-            law(own, p0, ax, ay, az);
-            law(own, p1, ax, ay, az);
-            law(own, p2, ax, ay, az);
-            law(own, p3, ax, ay, az);
-          }
-          for (; j < je; ++j) law(own, spos[j], ax, ay, az);
-          ax.flush();
-          ay.flush();
-          az.flush();
-        }
-      }
-    }
-    if (valid) {
-      const int i = (int)sval[tp];
-      F[i] = ax.total() * scale;
-      F[np + i] = ay.total() * scale;
-      F[2 * np + i] = az.total() * scale;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Symmetric cell-list force (cutoff mode, FP32c): every unordered pair within
-// the stencil once.  Work-group = one wave = (target cell c, forward set k):
-// k < 12 are the stencil rows (dy, dz) with dz > 0 or (dz == 0, dy > 0), each
-// 5 cells wide in x (contiguous in cell order); k = 12 is cells c+1, c+2 of
-// c's own row.  The cell's targets are taken 256 at a time (4 sub-blocks of
-// 64 in registers), the set's sources 64 at a time, each 64 x 64 tile by the
-// rotation scheme of the all-pairs kernel with the truncated law.  Partials:
-//   target  tp[k][3][n_pad]  (FP64, sorted position; k = 13: own-cell pairs)
-//   source  sp[slot][3][n_pad] (FP32), slot = 5 k + (source cell x - cx + 2)
-//           for k < 12, 60 + (source cell x - cx - 1) for k = 12,
-// so no two work-groups write the same entry; a work-group adds its later
-// target groups' reactions to its own earlier ones in order.  The pairs
-// inside one cell are done ordered (k_symcell_self), and k_symcell_reduce
-// adds every ion's entries in a fixed order.
-// ---------------------------------------------------------------------------
-constexpr int SC_MT = 4;            // target sub-blocks per wave
-constexpr int SC_TG = 64 * SC_MT;   // targets per group
-constexpr int SC_NK = 13;           // forward source sets per cell
-constexpr int SC_NS = 62;           // source partial slots
-
-// forward stencil row k < 12 -> (dy, dz)
-__device__ __forceinline__ void sc_row(int k, int& dy, int& dz) {
-  if (k < 2) {
-    dy = k + 1;
-    dz = 0;
-  } else {
-    dz = 1 + (k - 2) / 5;
-    dy = (k - 2) % 5 - 2;
-  }
-}
-
-// source range [s0, s1) of set k of cell (cx, cy, cz); empty if outside the grid
-__device__ __forceinline__ void sc_sources(const int* __restrict__ cell_start, int nx, int ny, int nz, int cx,
-                                           int cy, int cz, int k, int& s0, int& s1) {
-  s0 = s1 = 0;
-  if (k < 12) {
-    int dy, dz;
-    sc_row(k, dy, dz);
-    const int ry = cy + dy, rz = cz + dz;
-    if (ry < 0 || ry >= ny || rz >= nz) return;
-    const int row = nx * (ry + ny * rz);
-    s0 = cell_start[row + max(cx - 2, 0)];
-    s1 = cell_start[row + min(cx + 2, nx - 1) + 1];
-  } else {
-    if (cx + 1 >= nx) return;
-    const int c = cx + nx * (cy + ny * cz);
-    s0 = cell_start[c + 1];
-    s1 = cell_start[c + min(2, nx - 1 - cx) + 1];
-  }
-}
-
-template <int CUT>
-__global__ __launch_bounds__(64) void k_pair32_symcell(const DevParams* __restrict__ P,
-                                                       const float4* __restrict__ spos,
-                                                       const uint32_t* __restrict__ skey,
-                                                       const int* __restrict__ cell_start,
-                                                       double* __restrict__ tp, float* __restrict__ sp) {
-  const int nx = P->nx, ny = P->ny, nz = P->nz, np = P->n_pad;
-  const int c = blockIdx.x / SC_NK, k = blockIdx.x % SC_NK;
-  const int lane = threadIdx.x;
-  const int t0 = cell_start[c], t1 = cell_start[c + 1];
-  if (t0 == t1) return;
-  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-  int s0, s1;
-  sc_sources(cell_start, nx, ny, nz, cx, cy, cz, k, s0, s1);
-  const SymLaw L{P->a32, P->linv32, P->rc32, P->rs32, P->invw32};
-  const float4 park = make_float4(PARK32, PARK32, PARK32, 0.f);
-  for (int g0 = t0; g0 < t1; g0 += SC_TG) {
-    SymT<SC_MT> T;
-    float sx[SC_MT], sy[SC_MT], sz[SC_MT], ccx[SC_MT], ccy[SC_MT], ccz[SC_MT];
-#pragma unroll
-    for (int m = 0; m < SC_MT; ++m) {
-      const int t = g0 + m * 64 + lane;
-      const float4 q = (t < t1) ? spos[t] : park;
-      T.x[m] = q.x;
-      T.y[m] = q.y;
-      T.z[m] = q.z;
-      sx[m] = sy[m] = sz[m] = ccx[m] = ccy[m] = ccz[m] = 0.f;
-    }
-    for (int b0 = s0; b0 < s1; b0 += 64) {
-      const int j = b0 + lane;
-      const bool jv = j < s1;
-      const float4 q = jv ? spos[j] : park;
-      float bx = 0.f, by = 0.f, bz = 0.f;
-#pragma unroll
-      for (int m = 0; m < SC_MT; ++m) T.ax[m] = T.ay[m] = T.az[m] = 0.f;
-      sym_tile<SC_MT, CUT>(T, q.x, q.y, q.z, bx, by, bz, L);
-#pragma unroll
-      for (int m = 0; m < SC_MT; ++m) {
-        kahan_add(sx[m], ccx[m], T.ax[m]);
-        kahan_add(sy[m], ccy[m], T.ay[m]);
-        kahan_add(sz[m], ccz[m], T.az[m]);
-      }
-      if (jv) {
-        const int jx = (int)(skey[j] % (uint32_t)nx);
-        const int slot = (k < 12) ? 5 * k + (jx - cx + 2) : 60 + (jx - cx - 1);
-        float* o = sp + (size_t)slot * 3 * np + j;
-        if (g0 == t0) {
-          o[0] = bx;
-          o[np] = by;
-          o[2 * (size_t)np] = bz;
-        } else {
-          o[0] += bx;
-          o[np] += by;
-          o[2 * (size_t)np] += bz;
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < SC_MT; ++m) {
-      const int t = g0 + m * 64 + lane;
-      if (t < t1) {
-        double* o = tp + (size_t)k * 3 * np + t;
-        o[0] = (double)sx[m] - (double)ccx[m];
-        o[np] = (double)sy[m] - (double)ccy[m];
-        o[2 * (size_t)np] = (double)sz[m] - (double)ccz[m];
-      }
-    }
-  }
-}
-
-// pairs inside each cell, ordered (each target against every source of its
-// cell), into target slot 13; sources are work-group-uniform
-template <int SW>
-__global__ __launch_bounds__(256) void k_symcell_self(const DevParams* __restrict__ P, const float4* __restrict__ spos,
-                                                      const int* __restrict__ cell_start, double* __restrict__ tp) {
-  const CutLaw<float, SW> law(*P);
-  const int np = P->n_pad;
-  const int c = blockIdx.x;
-  const int t0 = cell_start[c], t1 = cell_start[c + 1];
-  for (int t = t0 + (int)threadIdx.x; t - (int)threadIdx.x < t1; t += 256) {
-    const bool tv = t < t1;
-    const float4 own = tv ? spos[t] : spos[t0];
-    Acc<float> ax, ay, az;
-    for (int jt = t0; jt < t1; jt += TJ) {
-      const int je = min(jt + TJ, t1);
-      for (int j = jt; j < je; ++j) law(own, spos[j], ax, ay, az);
-      ax.flush();
-      ay.flush();
-      az.flush();
-    }
-    if (tv) {
-      double* o = tp + (size_t)13 * 3 * np + t;
-      o[0] = ax.total();
-      o[np] = ay.total();
-      o[2 * (size_t)np] = az.total();
-    }
-  }
-}
-
-// F[ion] = pair_scale * (its 14 target partials + the source partials of every
-// work-group whose set holds it), fixed order
-__global__ __launch_bounds__(256) void k_symcell_reduce(const DevParams* __restrict__ P,
-                                                        const uint32_t* __restrict__ skey,
-                                                        const uint32_t* __restrict__ sval,
-                                                        const int* __restrict__ cell_start,
-                                                        const double* __restrict__ tp, const float* __restrict__ sp,
-                                                        double* __restrict__ F) {
-  const int nx = P->nx, ny = P->ny, nz = P->nz, np = P->n_pad;
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= cell_start[P->n_cells]) return;
-  const int c = (int)skey[p];
-  const int jx = c % nx, jy = (c / nx) % ny, jz = c / (nx * ny);
-  double s[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    double v = 0.0;
-    // every work-group (c, k) of a non-empty cell wrote its targets' entries,
-    // zero when its set is empty; slot 13 holds the own-cell pairs
-    for (int k = 0; k < 14; ++k) v += tp[((size_t)k * 3 + q) * np + p];
-    s[q] = v;
-  }
-  // reactions: work-groups (c', k) whose set k holds this ion's cell
-  for (int k = 0; k < SC_NK; ++k) {
-    int dy = 0, dz = 0, dlo, dhi;
-    if (k < 12) {
-      sc_row(k, dy, dz);
-      dlo = -2;
-      dhi = 2;
-    } else {
-      dlo = 1;
-      dhi = 2;
-    }
-    const int cy = jy - dy, cz = jz - dz;
-    if (cy < 0 || cy >= ny || cz < 0 || cz >= nz) continue;
-    for (int d = dlo; d <= dhi; ++d) {
-      const int cx = jx - d;
-      if (cx < 0 || cx >= nx) continue;
-      const int cc = cx + nx * (cy + ny * cz);
-      if (cell_start[cc] == cell_start[cc + 1]) continue;
-      const int slot = (k < 12) ? 5 * k + (d + 2) : 60 + (d - 1);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) s[q] += (double)sp[((size_t)slot * 3 + q) * np + p];
-    }
-  }
-  const int i = (int)sval[p];
-  const double sc = P->pair_scale;
-  F[i] = s[0] * sc;
-  F[np + i] = s[1] * sc;
-  F[2 * (size_t)np + i] = s[2] * sc;
-}
-
-// Rows of listed ions in cutoff mode: kept sources from the cell stencil of the
-// ion's cell (FP64 position X) plus the other listed ions `others`.
-template <class T>
-__global__ __launch_bounds__(256) void k_rows_cell(const DevParams* __restrict__ P, const double* __restrict__ X,
-                                                   const typename Pos<T>::V* __restrict__ spos,
-                                                   const int* __restrict__ cell_start, const int* __restrict__ n_flag,
-                                                   const typename Pos<T>::V* __restrict__ own_list,
-                                                   const typename Pos<T>::V* __restrict__ others,
-                                                   const uint32_t* __restrict__ flags, const int* __restrict__ list_ion,
-                                                   uint32_t need_mask, double* __restrict__ rowF) {
-  __shared__ double lds[4];
-  const int nf = n_flag[0];
-  const int np = P->n_pad;
-  const int nx = P->nx, ny = P->ny, nz = P->nz;
-  for (int s = blockIdx.x; s < nf; s += gridDim.x) {
-    const int ion = list_ion[s];
-    if (need_mask != 0u && (flags[ion] & need_mask) == 0u) continue;
-    const typename Pos<T>::V own = own_list[s];
-    const int c = cell_of(*P, X[ion], X[np + ion], X[2 * np + ion]);
-    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-    Acc<T> ax, ay, az;
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z) {
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
-        const int row = nx * (y + ny * z);
-        const int j0 = cell_start[row + max(cx - 2, 0)];
-        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
-        for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) pair_acc<T>(*P, own, spos[j], ax, ay, az);
-        ax.flush();
-        ay.flush();
-        az.flush();
-      }
-    }
-    for (int j = threadIdx.x; j < nf; j += 256) pair_acc<T>(*P, own, others[j], ax, ay, az);
-    ax.flush();
-    ay.flush();
-    az.flush();
-    const double fx = block_sum256(ax.total(), lds);
-    const double fy = block_sum256(ay.total(), lds);
-    const double fz = block_sum256(az.total(), lds);
-    if (threadIdx.x == 0) {
-      rowF[3 * s + 0] = fx * P->pair_scale;
-      rowF[3 * s + 1] = fy * P->pair_scale;
-      rowF[3 * s + 2] = fz * P->pair_scale;
-    }
-  }
-}
-
-// Share of the listed (flagged) sources in the force on the binned (kept)
-// ions, cutoff mode.  Targets run in cell order, so a wave's 64 targets are
-// close together: a listed source is evaluated only if it lies within the
-// cutoff (plus a relative margin of 1e-5) of the wave's bounding box, decided
-// 64 sources at a time by a ballot; skipped sources are beyond the cutoff of
-// every target, where the law gives exactly zero.
-// out[ion] = (add ? out[ion] : 0) + sum * pair_scale.
-template <class T, int SW>
-__global__ __launch_bounds__(256) void k_corr_cell(const DevParams* __restrict__ P,
-                                                   const typename Pos<T>::V* __restrict__ spos,
-                                                   const uint32_t* __restrict__ sval,
-                                                   const int* __restrict__ cell_start, const int* __restrict__ n_flag,
-                                                   const typename Pos<T>::V* __restrict__ list, int add,
-                                                   double* __restrict__ out) {
-  using PV = typename Pos<T>::V;
-  const CutLaw<T, SW> law(*P);
-  const int np = P->n_pad;
-  const int nbin = cell_start[P->n_cells];
-  const int wave0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 256 + (threadIdx.x & ~63)));
-  if (wave0 >= nbin) return;
-  const int lane = threadIdx.x & 63;
-  const int p = wave0 + lane;
-  const bool valid = p < nbin;
-  const PV own = valid ? spos[p] : spos[wave0];
-  T lox = own.x, loy = own.y, loz = own.z, hix = own.x, hiy = own.y, hiz = own.z;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    lox = min(lox, __shfl_xor(lox, o, 64));
-    loy = min(loy, __shfl_xor(loy, o, 64));
-    loz = min(loz, __shfl_xor(loz, o, 64));
-    hix = max(hix, __shfl_xor(hix, o, 64));
-    hiy = max(hiy, __shfl_xor(hiy, o, 64));
-    hiz = max(hiz, __shfl_xor(hiz, o, 64));
-  }
-  const T rcm = (sizeof(T) == 4 ? (T)P->rc32 : (T)P->rc64) * (T)1.00001;
-  const T rcm2 = rcm * rcm;
-  Acc<T> ax, ay, az;
-  int cnt = 0;
-  const int nf = n_flag[0];
-  for (int s0 = 0; s0 < nf; s0 += 64) {
-    const int s = s0 + lane;
-    const PV q = (s < nf) ? list[s] : Pos<T>::park();
-    const T dx = max(max(lox - q.x, q.x - hix), (T)0);
-    const T dy = max(max(loy - q.y, q.y - hiy), (T)0);
-    const T dz = max(max(loz - q.z, q.z - hiz), (T)0);
-    uint64_t m = __ballot(s < nf && dx * dx + dy * dy + dz * dz <= rcm2);
-    while (m) {
-      const int l = __builtin_ctzll(m);
-      m &= m - 1;
-      PV qs;
-      qs.x = __shfl(q.x, l, 64);
-      qs.y = __shfl(q.y, l, 64);
-      qs.z = __shfl(q.z, l, 64);
-      qs.w = 0;
-      law(own, qs, ax, ay, az);
-      if (++cnt == TJ) {
-        ax.flush();
-        ay.flush();
-        az.flush();
-        cnt = 0;
-      }
-    }
-  }
-  ax.flush();
-  ay.flush();
-  az.flush();
-  if (valid) {
-    const int i = (int)sval[p];
-    const double sc = P->pair_scale;
-    out[i] = (add ? out[i] : 0.0) + ax.total() * sc;
-    out[np + i] = (add ? out[np + i] : 0.0) + ay.total() * sc;
-    out[2 * np + i] = (add ? out[2 * np + i] : 0.0) + az.total() * sc;
-  }
-}
-
-// in-range and candidate pair counts over the current cell list (the same
-// distance test as k_pair_cell); integer atomics, so the totals are exact
-template <class T>
-__global__ __launch_bounds__(256) void k_cell_pair_count(const DevParams* __restrict__ P,
-                                                         const typename Pos<T>::V* __restrict__ spos,
-                                                         const int* __restrict__ cell_start,
-                                                         const int* __restrict__ items, const int* __restrict__ n_items,
-                                                         unsigned long long* __restrict__ counts) {
-  const int lane = threadIdx.x & 63;
-  const int nx = P->nx, ny = P->ny, nz = P->nz;
-  const int nit = n_items[0];
-  const int nw = gridDim.x * 4;
-  unsigned long long in_range = 0, cand = 0;
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nit; it += nw) {
-    const int packed = items[it];
-    const int c = packed & 0xFFFF, k = packed >> 16;
-    const int tp = cell_start[c] + k * 64 + lane;
-    const bool valid = tp < cell_start[c + 1];
-    const typename Pos<T>::V own = valid ? spos[tp] : Pos<T>::park();
-    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-    for (int z = max(cz - 2, 0); z <= min(cz + 2, nz - 1); ++z)
-      for (int y = max(cy - 2, 0); y <= min(cy + 2, ny - 1); ++y) {
-        const int row = nx * (y + ny * z);
-        const int j0 = cell_start[row + max(cx - 2, 0)];
-        const int j1 = cell_start[row + min(cx + 2, nx - 1) + 1];
-        for (int j = j0; j < j1; ++j) {
-          if (!valid || j == tp) continue;
-          const typename Pos<T>::V q = spos[j];
-          bool in;
-          if constexpr (sizeof(T) == 4) {
-            const float dx = q.x - own.x, dy = q.y - own.y, dz = q.z - own.z;
-            const float r = __builtin_amdgcn_sqrtf(
-                __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, EPS32))));
-            in = r < P->rc32;
-          } else {
-            const double dx = q.x - own.x, dy = q.y - own.y, dz = q.z - own.z;
-            in = sqrt(fma(dx, dx, fma(dy, dy, fma(dz, dz, EPS64)))) < P->rc64;
-          }
-          in_range += in ? 1 : 0;
-          cand += 1;
-        }
-      }
-  }
-  atomicAdd(&counts[0], in_range);
-  atomicAdd(&counts[1], cand);
-}
+#include "od_kernels_pair.inc"
+#include "od_kernels_step.inc"
+#include "od_kernels_trap.inc"
+#include "od_kernels_cell.inc"
 
 // ---------------------------------------------------------------------------
 // context
