@@ -1212,490 +1212,7 @@ extern "C" const char* od_last_error(void) { return g_err.c_str(); }
 
 extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950, ABI " "1"; }
 
-// ---------------------------------------------------------------------------
-// Batched parameter sweeps: many independent simulations with uneven ion and
-// dust counts advanced together.  Their ions are concatenated (each padded to a
-// multiple of 1024), a block -> simulation table selects each 256-ion block's
-// parameter record, and the all-pairs work of all simulations is one launch
-// over a list of items (simulation, 512 targets, source chunk) so small and
-// large simulations fill the GPU together.  The step is the same sequence of
-// kernels as a single simulation's, each launched once for all simulations,
-// and can be replayed as a captured hipGraph.
-// ---------------------------------------------------------------------------
-constexpr int BATCH_SRC_CHUNK = 8192;  // sources per pair item
-
-struct PairItem {
-  int sim;       // simulation
-  int tgt;       // first of the 512 targets (global ion index)
-  int src0, src1;  // source range (global ion indices)
-};
-
-template <class T>
-__global__ __launch_bounds__(256) void k_pair_items(const DevParams* __restrict__ Pall,
-                                                    const PairItem* __restrict__ items,
-                                                    const typename Pos<T>::V* __restrict__ pj,
-                                                    double* __restrict__ part) {
-  const PairItem it = items[blockIdx.x];
-  const DevParams* P = Pall + it.sim;
-  const typename Pos<T>::V a = pj[it.tgt + threadIdx.x];
-  const typename Pos<T>::V b = pj[it.tgt + 256 + threadIdx.x];
-  Acc<T> ax0, ay0, az0, ax1, ay1, az1;
-  const PlainLaw<T> law(*P);  // batched sweeps run all-pairs forces
-  for (int jt = it.src0; jt < it.src1; jt += TJ) {
-#pragma unroll 8
-    for (int j = jt; j < jt + TJ; ++j) {
-      const typename Pos<T>::V p = pj[j];
-      law(a, p, ax0, ay0, az0);
-      law(b, p, ax1, ay1, az1);
-    }
-    ax0.flush();
-    ay0.flush();
-    az0.flush();
-    ax1.flush();
-    ay1.flush();
-    az1.flush();
-  }
-  double* o = part + (size_t)blockIdx.x * 3 * 512;
-  o[threadIdx.x] = ax0.total();
-  o[512 + threadIdx.x] = ay0.total();
-  o[1024 + threadIdx.x] = az0.total();
-  o[256 + threadIdx.x] = ax1.total();
-  o[512 + 256 + threadIdx.x] = ay1.total();
-  o[1024 + 256 + threadIdx.x] = az1.total();
-}
-
-// F = pair_scale(sim) * sum of the ion's items (fixed order); group_items[g] =
-// {first item, count} of the 512-ion group g
-__global__ __launch_bounds__(256) void k_items_reduce(const DevParams* __restrict__ Pall,
-                                                      const int* __restrict__ blk_sim,
-                                                      const int2* __restrict__ group_items,
-                                                      const double* __restrict__ part, int n_tot,
-                                                      double* __restrict__ F) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_tot) return;
-  const DevParams* P = Pall + blk_sim[blockIdx.x];
-  const int2 gi = group_items[i >> 9];
-  const int off = i & 511;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double s = 0.0;
-    for (int k = 0; k < gi.y; ++k) s += part[((size_t)(gi.x + k) * 3 + c) * 512 + off];
-    F[(size_t)c * n_tot + i] = s * P->pair_scale;
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_batch_load_pos(const DevParams* __restrict__ Pall,
-                                                        const int* __restrict__ blk_sim, const double* __restrict__ X,
-                                                        typename Pos<T>::V* __restrict__ pi,
-                                                        typename Pos<T>::V* __restrict__ pj) {
-  const DevParams* P = Pall + blk_sim[blockIdx.x];
-  const int np = P->stride;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const typename Pos<T>::V p = (i - P->ion_off < P->n) ? Pos<T>::make(X[i], X[np + i], X[2 * np + i], P->pos_scale)
-                                                      : Pos<T>::park();
-  pi[i] = p;
-  pj[i] = p;
-}
-
-struct od_batch {
-  int n_sims = 0, dev = 0, prec = OD_PREC_FP32C;
-  hipStream_t stream = nullptr;
-  std::vector<od_config> cfgs;
-  std::vector<int> off, npad;
-  int n_tot = 0, nb = 0, n_items = 0, rows_grid = 1024, rec_cap = 128;
-  size_t pos_bytes = 16;
-  DevParams* dP = nullptr;
-  int* blk_sim = nullptr;
-  PairItem* items = nullptr;
-  int2* group_items = nullptr;
-  double *X = nullptr, *V = nullptr, *F = nullptr, *q = nullptr, *partials = nullptr, *rowF = nullptr,
-         *rowD = nullptr, *part = nullptr;
-  void *pi = nullptr, *pj = nullptr, *list_old = nullptr, *list_new = nullptr;
-  uint32_t *flags = nullptr, *st = nullptr;
-  uint8_t* face = nullptr;
-  int *block_cnt = nullptr, *block_off = nullptr, *n_flag = nullptr, *slot_of = nullptr, *list_ion = nullptr,
-      *err = nullptr;
-  od_dust_record* rec = nullptr;
-  int64_t host_step = 0;
-  bool stale = true;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  int graph_steps = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double pair_ms = 0.0;
-  int64_t pair_launches = 0, pairs_per_launch = 0;
-  // symmetric FP32 pair kernel over all simulations (1024-ion blocks)
-  int sym = 0, sym_ch = 1, n_wg = 0, n_blocks = 0, sym_var = 0;
-  std::vector<int> blk_end_host;
-  int2* wg_tab = nullptr;
-  int *wg_base = nullptr, *blk_begin = nullptr, *blk_end = nullptr;
-  float* blk_a = nullptr;
-  double *blk_scale = nullptr, *sym_tp = nullptr, *sym_dg = nullptr;
-  float* sym_sp = nullptr;
-};
-
-static void free_batch(od_batch* b) {
-  if (!b) return;
-  hipSetDevice(b->dev);
-  if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  void* bufs[] = {b->dP, b->blk_sim, b->items, b->group_items, b->X, b->V, b->F, b->q, b->partials, b->rowF,
-                  b->rowD, b->part, b->pi, b->pj, b->list_old, b->list_new, b->flags, b->st, b->face, b->block_cnt,
-                  b->block_off, b->n_flag, b->slot_of, b->list_ion, b->err, b->rec, b->wg_tab, b->wg_base,
-                  b->blk_begin, b->blk_end, b->blk_a, b->blk_scale, b->sym_tp, b->sym_dg, b->sym_sp};
-  for (void* p : bufs)
-    if (p) hipFree(p);
-  if (b->stream) hipStreamDestroy(b->stream);
-  delete b;
-}
-
-extern "C" int od_batch_create(const od_config* cfgs, int32_t n_sims, od_batch** out) {
-  if (!out || !cfgs) return set_err(OD_ERR_ARG, "null argument");
-  *out = nullptr;
-  if (n_sims < 1 || n_sims > 4096) return set_err(OD_ERR_ARG, "n_sims %d outside [1, 4096]", n_sims);
-  for (int s = 0; s < n_sims; ++s) {
-    CHKRET(check_config(&cfgs[s]));
-    if (cfgs[s].precision != cfgs[0].precision || cfgs[s].device != cfgs[0].device)
-      return set_err(OD_ERR_ARG, "all simulations of a batch share precision and device");
-    if (cfgs[s].cutoff > 0.0) return set_err(OD_ERR_ARG, "batched sweeps run all-pairs forces (no cutoff)");
-  }
-  int ndev = 0;
-  hipError_t e = hipGetDeviceCount(&ndev);
-  if (e != hipSuccess || ndev == 0) return set_err(OD_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
-  if (cfgs[0].device < 0 || cfgs[0].device >= ndev) return set_err(OD_ERR_ARG, "device %d out of range", cfgs[0].device);
-  od_batch* b = new od_batch();
-  auto fail = [&](int r) {
-    free_batch(b);
-    return r;
-  };
-  b->n_sims = n_sims;
-  b->dev = cfgs[0].device;
-  b->prec = cfgs[0].precision;
-  b->pos_bytes = (b->prec == OD_PREC_FP32C) ? sizeof(float4) : sizeof(double4);
-  b->cfgs.assign(cfgs, cfgs + n_sims);
-  int64_t tot = 0;
-  for (int s = 0; s < n_sims; ++s) {
-    const int np = (cfgs[s].n_ions + 1023) / 1024 * 1024;
-    b->off.push_back((int)tot);
-    b->npad.push_back(np);
-    tot += np;
-  }
-  if (tot > (1 << 28)) return fail(set_err(OD_ERR_ARG, "batch too large (%lld ions)", (long long)tot));
-  b->n_tot = (int)tot;
-  b->nb = b->n_tot / 256;
-  // parameter records and tables
-  std::vector<DevParams> hp(n_sims);
-  std::vector<int> blk_sim(b->nb);
-  std::vector<PairItem> items;
-  std::vector<int2> groups(b->n_tot / 512);
-  for (int s = 0; s < n_sims; ++s) {
-    fill_params(&cfgs[s], b->npad[s], &hp[s]);
-    hp[s].ion_off = b->off[s];
-    hp[s].stride = b->n_tot;
-    hp[s].blk0 = b->off[s] / 256;
-    hp[s].blk1 = (b->off[s] + b->npad[s]) / 256;
-    hp[s].q_off = s * OD_MAX_DUST;
-    for (int k = hp[s].blk0; k < hp[s].blk1; ++k) blk_sim[k] = s;
-    for (int t = 0; t < b->npad[s]; t += 512) {
-      groups[(b->off[s] + t) / 512] = make_int2((int)items.size(), 0);
-      for (int j = 0; j < b->npad[s]; j += BATCH_SRC_CHUNK) {
-        PairItem it;
-        it.sim = s;
-        it.tgt = b->off[s] + t;
-        it.src0 = b->off[s] + j;
-        it.src1 = b->off[s] + std::min(b->npad[s], j + BATCH_SRC_CHUNK);
-        items.push_back(it);
-        groups[(b->off[s] + t) / 512].y += 1;
-      }
-    }
-  }
-  b->n_items = (int)items.size();
-  b->sym = (b->prec == OD_PREC_FP32C && cfgs[0].pair_mode != OD_PAIR_ROWS) ? 1 : 0;
-  const size_t np = b->n_tot;
-  int rc = OD_OK;
-  if (hipSetDevice(b->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice failed"));
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(set_err(OD_ERR_HIP, "hipStreamCreate failed"));
-  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
-    return fail(set_err(OD_ERR_HIP, "hipEventCreate failed"));
-  for (int s = 0; s < n_sims; ++s) b->pairs_per_launch += (int64_t)cfgs[s].n_ions * (cfgs[s].n_ions - 1);
-  if ((rc = dalloc(&b->dP, n_sims * sizeof(DevParams))) || (rc = dalloc(&b->blk_sim, b->nb * 4)) ||
-      (rc = dalloc(&b->items, items.size() * sizeof(PairItem))) ||
-      (rc = dalloc(&b->group_items, groups.size() * sizeof(int2))) || (rc = dalloc(&b->X, 3 * np * 8)) ||
-      (rc = dalloc(&b->V, 3 * np * 8)) || (rc = dalloc(&b->F, 3 * np * 8)) ||
-      (rc = dalloc(&b->q, (size_t)n_sims * OD_MAX_DUST * 8)) ||
-      (rc = dalloc(&b->partials, (size_t)b->nb * OD_MAX_DUST * 8 * 8)) || (rc = dalloc(&b->rowF, 3 * np * 8)) ||
-      (rc = dalloc(&b->rowD, 3 * np * 8)) || (rc = dalloc(&b->part, b->sym ? 8 : (size_t)b->n_items * 3 * 512 * 8)) ||
-      (rc = dalloc(&b->pi, np * b->pos_bytes)) || (rc = dalloc(&b->pj, np * b->pos_bytes)) ||
-      (rc = dalloc(&b->list_old, np * b->pos_bytes)) || (rc = dalloc(&b->list_new, np * b->pos_bytes)) ||
-      (rc = dalloc(&b->flags, np * 4)) || (rc = dalloc(&b->st, 16)) || (rc = dalloc(&b->face, np)) ||
-      (rc = dalloc(&b->block_cnt, b->nb * 4)) || (rc = dalloc(&b->block_off, b->nb * 4)) ||
-      (rc = dalloc(&b->n_flag, 4)) || (rc = dalloc(&b->slot_of, np * 4)) || (rc = dalloc(&b->list_ion, np * 4)) ||
-      (rc = dalloc(&b->err, 4)) ||
-      (rc = dalloc(&b->rec, (size_t)b->rec_cap * n_sims * OD_MAX_DUST * sizeof(od_dust_record))))
-    return fail(rc);
-  std::vector<double> q0((size_t)n_sims * OD_MAX_DUST, 0.0);
-  for (int s = 0; s < n_sims; ++s)
-    for (int p = 0; p < cfgs[s].n_dust; ++p) q0[(size_t)s * OD_MAX_DUST + p] = cfgs[s].dust_q0[p];
-  uint32_t st0[4] = {0, 0, 0, 0};
-  if (hipMemcpy(b->dP, hp.data(), n_sims * sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->blk_sim, blk_sim.data(), b->nb * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->items, items.data(), items.size() * sizeof(PairItem), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->group_items, groups.data(), groups.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->q, q0.data(), q0.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->st, st0, sizeof(st0), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(b->X, 0, 3 * np * 8) != hipSuccess || hipMemset(b->V, 0, 3 * np * 8) != hipSuccess ||
-      hipMemset(b->F, 0, 3 * np * 8) != hipSuccess || hipMemset(b->err, 0, 4) != hipSuccess ||
-      hipMemset(b->n_flag, 0, 4) != hipSuccess || hipMemset(b->flags, 0, np * 4) != hipSuccess ||
-      hipMemset(b->face, 0, np) != hipSuccess ||
-      hipMemset(b->rec, 0, (size_t)b->rec_cap * n_sims * OD_MAX_DUST * sizeof(od_dust_record)) != hipSuccess)
-    return fail(set_err(OD_ERR_HIP, "batch initialisation failed"));
-  if (b->sym) {
-    // triangular schedule over every simulation's 1024-ion blocks
-    const int NB = b->n_tot / SYM_TB;
-    std::vector<int> bb(NB), be(NB);
-    std::vector<float> ba(NB);
-    std::vector<double> bs(NB);
-    for (int s = 0; s < n_sims; ++s)
-      for (int X = b->off[s] / SYM_TB; X < (b->off[s] + b->npad[s]) / SYM_TB; ++X) {
-        bb[X] = b->off[s] / SYM_TB;
-        be[X] = (b->off[s] + b->npad[s]) / SYM_TB;
-        ba[X] = hp[s].a32;
-        bs[X] = hp[s].pair_scale;
-      }
-    // ordered pairs of real ions the symmetric launch covers (different blocks)
-    b->pairs_per_launch = 0;
-    for (int s = 0; s < n_sims; ++s) {
-      const int64_t n = cfgs[s].n_ions;
-      int64_t diag = 0;
-      for (int64_t x0 = 0; x0 < n; x0 += SYM_TB) {
-        const int64_t m = std::min<int64_t>(SYM_TB, n - x0);
-        diag += m * (m - 1);
-      }
-      b->pairs_per_launch += n * (n - 1) - diag;
-    }
-    b->blk_end_host = be;
-    if ((rc = sym_build(b, be, sym_default_chunk(be, cfgs[0].sym_chunk))) || (rc = dalloc(&b->blk_begin, NB * 4)) ||
-        (rc = dalloc(&b->blk_end, NB * 4)) || (rc = dalloc(&b->blk_a, NB * 4)) ||
-        (rc = dalloc(&b->blk_scale, NB * 8)) || (rc = dalloc(&b->sym_dg, 3 * np * 8)))
-      return fail(rc);
-    if (hipMemcpy(b->blk_begin, bb.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->blk_end, be.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->blk_a, ba.data(), NB * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->blk_scale, bs.data(), NB * 8, hipMemcpyHostToDevice) != hipSuccess)
-      return fail(set_err(OD_ERR_HIP, "batch initialisation failed"));
-    b->sym_var = sym_default_variant(b->n_wg, cfgs[0].sym_tile);
-  }
-  *out = b;
-  return OD_OK;
-}
-
-extern "C" int od_batch_destroy(od_batch* b) {
-  free_batch(b);
-  return OD_OK;
-}
-
-extern "C" int od_batch_set_ions(od_batch* b, int32_t sim, const double* x, const double* v, int64_t n) {
-  if (!b || !x || !v) return set_err(OD_ERR_ARG, "null argument");
-  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
-  if (n != b->cfgs[sim].n_ions)
-    return set_err(OD_ERR_ARG, "simulation %d holds %d ions, got %lld", sim, b->cfgs[sim].n_ions, (long long)n);
-  for (int64_t i = 0; i < 3 * n; ++i)
-    if (!std::isfinite(x[i]) || !std::isfinite(v[i])) return set_err(OD_ERR_NONFINITE, "non-finite ion state");
-  HIPCHK(hipSetDevice(b->dev));
-  const size_t np = b->n_tot;
-  for (int k = 0; k < 3; ++k) {
-    std::vector<double> cx(n), cv(n);
-    for (int64_t i = 0; i < n; ++i) {
-      cx[i] = x[3 * i + k];
-      cv[i] = v[3 * i + k];
-    }
-    HIPCHK(hipMemcpy(b->X + k * np + b->off[sim], cx.data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->V + k * np + b->off[sim], cv.data(), n * 8, hipMemcpyHostToDevice));
-  }
-  b->stale = true;
-  return OD_OK;
-}
-
-extern "C" int od_batch_get_ions(od_batch* b, int32_t sim, double* x, double* v) {
-  if (!b) return set_err(OD_ERR_ARG, "null batch");
-  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
-  HIPCHK(hipSetDevice(b->dev));
-  const int n = b->cfgs[sim].n_ions;
-  const size_t np = b->n_tot;
-  std::vector<double> c(n);
-  for (int k = 0; k < 3; ++k) {
-    if (x) {
-      HIPCHK(hipMemcpy(c.data(), b->X + k * np + b->off[sim], n * 8, hipMemcpyDeviceToHost));
-      for (int i = 0; i < n; ++i) x[3 * (size_t)i + k] = c[i];
-    }
-    if (v) {
-      HIPCHK(hipMemcpy(c.data(), b->V + k * np + b->off[sim], n * 8, hipMemcpyDeviceToHost));
-      for (int i = 0; i < n; ++i) v[3 * (size_t)i + k] = c[i];
-    }
-  }
-  return OD_OK;
-}
-
-template <class T>
-static void batch_pair(od_batch* b, bool timed, double* Fout) {
-  using PV = typename Pos<T>::V;
-  if (b->sym) {
-    const float linv = (float)(1.0 / 1.4426950408889634);
-    if (timed) hipEventRecord(b->ev0, b->stream);
-    if (b->n_wg > 0)
-      hipLaunchKernelGGL(sym_variants[b->sym_var].fn, dim3(b->n_wg), dim3(SYM_TB / sym_variants[b->sym_var].mt), 0,
-                         b->stream, (const float4*)b->pj, b->n_blocks,
-                         b->sym_ch, (const int2*)b->wg_tab, (const int*)b->wg_base, 0.0f, linv,
-                         (const int*)b->blk_end, (const float*)b->blk_a, b->sym_tp, b->sym_sp);
-    if (timed) hipEventRecord(b->ev1, b->stream);
-    hipLaunchKernelGGL(k_pair32_diag, dim3(b->n_tot / 256), dim3(256), 0, b->stream, (const float4*)b->pj, b->n_tot,
-                       0.0f, linv, (const float*)b->blk_a, b->sym_dg);
-    hipLaunchKernelGGL(k_sym_reduce, dim3(b->n_tot / 256), dim3(256), 0, b->stream, b->sym_dg, b->sym_tp,
-                       b->sym_sp, b->wg_base, b->n_blocks, b->sym_ch, b->n_tot, 0.0, (const int*)b->blk_begin,
-                       (const double*)b->blk_scale, Fout);
-    return;
-  }
-  if (timed) hipEventRecord(b->ev0, b->stream);
-  hipLaunchKernelGGL(k_pair_items<T>, dim3(b->n_items), dim3(256), 0, b->stream, b->dP, b->items, (const PV*)b->pj,
-                     b->part);
-  if (timed) hipEventRecord(b->ev1, b->stream);
-  hipLaunchKernelGGL(k_items_reduce, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->group_items,
-                     b->part, b->n_tot, Fout);
-}
-
-template <class T>
-static int batch_step_t(od_batch* b, bool timed) {
-  using PV = typename Pos<T>::V;
-  const int nb = b->nb;
-  hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X, b->V, b->F, b->q,
-                     b->st, (PV*)b->pi, (PV*)b->pj, b->flags, b->block_cnt, b->err);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, b->stream, b->block_cnt, nb, b->block_off, b->n_flag);
-  hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, b->stream, b->flags, b->block_cnt, b->block_off, b->n_tot,
-                     (const PV*)b->pi, b->slot_of, b->list_ion, (PV*)b->list_old);
-  batch_pair<T>(b, timed, b->F);
-  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(RowShape<T>::TB), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pi,
-                     b->n_flag, (const PV*)b->list_old, b->flags, b->list_ion, (uint32_t)((1u << OD_MAX_DUST) - 1u),
-                     b->rowD);
-  hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->X, b->V,
-                     b->F, b->q, b->st, (const PV*)b->pi, (PV*)b->pj, b->flags, b->n_flag, b->slot_of,
-                     (const PV*)b->list_old, (PV*)b->list_new, b->face, b->rowD, (const double*)nullptr,
-                     b->partials);
-  hipLaunchKernelGGL(k_rows<T>, dim3(b->rows_grid), dim3(RowShape<T>::TB), 0, b->stream, b->dP, b->blk_sim, (const PV*)b->pj,
-                     b->n_flag, (const PV*)b->list_new, b->flags, b->list_ion, 0u, b->rowF);
-  hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->block_off, nb, b->F,
-                     (const PV*)b->pj, b->flags, b->n_flag, (const PV*)b->list_new, b->slot_of, b->rowF, 0);
-  hipLaunchKernelGGL(k_dust, dim3(b->n_sims), dim3(256), 0, b->stream, b->dP, b->partials, b->q, b->st, b->rec,
-                     (int)OD_MAX_DUST, 0);
-  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->st);
-  return OD_OK;
-}
-
-static int batch_step(od_batch* b, bool timed) {
-  return (b->prec == OD_PREC_FP32C) ? batch_step_t<float>(b, timed) : batch_step_t<double>(b, timed);
-}
-
-template <class T>
-static int batch_forces_t(od_batch* b) {
-  using PV = typename Pos<T>::V;
-  hipLaunchKernelGGL(k_batch_load_pos<T>, dim3(b->nb), dim3(256), 0, b->stream, b->dP, b->blk_sim, b->X,
-                     (PV*)b->pi, (PV*)b->pj);
-  batch_pair<T>(b, false, b->F);
-  return OD_OK;
-}
-
-extern "C" int od_batch_run(od_batch* b, int64_t nsteps, od_dust_record* out, int32_t flags) {
-  if (!b) return set_err(OD_ERR_ARG, "null batch");
-  if (nsteps < 0) return set_err(OD_ERR_ARG, "negative step count");
-  if (b->host_step + nsteps > 0xFFFFFFFFll) return set_err(OD_ERR_ARG, "step counter would exceed u32");
-  HIPCHK(hipSetDevice(b->dev));
-  if (b->stale) {
-    CHKRET((b->prec == OD_PREC_FP32C) ? batch_forces_t<float>(b) : batch_forces_t<double>(b));
-    HIPCHK(hipGetLastError());
-    b->stale = false;
-  }
-  const bool profile = (flags & OD_RUN_PROFILE_PAIR) != 0;
-  const bool graph = (flags & OD_RUN_GRAPH) != 0 && !profile;
-  int64_t done = 0;
-  const size_t rec_step = (size_t)b->n_sims * OD_MAX_DUST;
-  while (done < nsteps) {
-    const int chunk = (int)std::min<int64_t>(b->rec_cap, nsteps - done);
-    uint32_t base = (uint32_t)b->host_step;
-    HIPCHK(hipMemcpyAsync(b->st + 1, &base, 4, hipMemcpyHostToDevice, b->stream));
-    int left = chunk;
-    if (graph && chunk >= 4) {
-      const int gs = std::min(chunk, 16);
-      if (!(b->gexec && b->graph_steps == gs)) {
-        if (b->gexec) hipGraphExecDestroy(b->gexec);
-        if (b->graph) hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = OD_OK;
-        for (int s = 0; s < gs && rc == OD_OK; ++s) rc = batch_step(b, false);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc != OD_OK) return rc;
-        if (e != hipSuccess) return set_err(OD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-        b->graph_steps = gs;
-      }
-      while (left >= gs) {
-        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-        left -= gs;
-      }
-    }
-    for (; left > 0; --left) {
-      CHKRET(batch_step(b, profile));
-      HIPCHK(hipGetLastError());
-      if (profile) {
-        HIPCHK(hipEventSynchronize(b->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        b->pair_ms += ms;
-        b->pair_launches += 1;
-      }
-    }
-    if (out)
-      HIPCHK(hipMemcpyAsync(out + done * rec_step, b->rec, (size_t)chunk * rec_step * sizeof(od_dust_record),
-                            hipMemcpyDeviceToHost, b->stream));
-    int herr = 0;
-    HIPCHK(hipMemcpyAsync(&herr, b->err, 4, hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    b->host_step += chunk;
-    done += chunk;
-    if (herr) return set_err(OD_ERR_NONFINITE, "non-finite ion state in the batch before step %lld",
-                             (long long)b->host_step);
-  }
-  return OD_OK;
-}
-
-extern "C" int od_batch_get_step_info(od_batch* b, int32_t sim, uint32_t* flags, uint8_t* face) {
-  if (!b) return set_err(OD_ERR_ARG, "null batch");
-  if (sim < 0 || sim >= b->n_sims) return set_err(OD_ERR_ARG, "simulation %d out of range", sim);
-  HIPCHK(hipSetDevice(b->dev));
-  const int n = b->cfgs[sim].n_ions;
-  if (flags) HIPCHK(hipMemcpy(flags, b->flags + b->off[sim], (size_t)n * 4, hipMemcpyDeviceToHost));
-  if (face) HIPCHK(hipMemcpy(face, b->face + b->off[sim], (size_t)n, hipMemcpyDeviceToHost));
-  return OD_OK;
-}
-
-extern "C" int od_batch_pair_timing(od_batch* b, double* total_ms, int64_t* launches, int64_t* pairs_per_launch,
-                                    int32_t reset) {
-  if (!b) return set_err(OD_ERR_ARG, "null batch");
-  if (total_ms) *total_ms = b->pair_ms;
-  if (launches) *launches = b->pair_launches;
-  if (pairs_per_launch) *pairs_per_launch = b->pairs_per_launch;
-  if (reset) {
-    b->pair_ms = 0.0;
-    b->pair_launches = 0;
-  }
-  return OD_OK;
-}
+#include "od_batch.inc"
 
 // Autotuner of the symmetric pair evaluation (kernel, diagonal, reduction) on
 // the current positions.  Stage 1 times each candidate number of source blocks
