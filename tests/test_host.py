@@ -331,3 +331,29 @@ def test_field_driven_vz_density_is_exponentially_modified_gaussian():
         # the top face's flux weights the same density by v (opendust.py:673-689)
         flux = host._flux_field_driven(v_fl, v_T)
         assert np.allclose(flux(v), v * ref, rtol=1e-9, atol=1e-12 * np.max(np.abs(v * ref)))
+
+
+def test_blocking_error_of_the_validation_scripts():
+    """The blocking estimate the validation scripts quote: white noise gives
+    sigma / sqrt(n); an AR(1) series gives the analytic standard error of its
+    mean, sigma / sqrt(n) * sqrt((1 + phi) / (1 - phi)), where 10 fixed blocks of
+    a short window would understate it."""
+    import importlib.util
+
+    here = os.path.dirname(__file__)
+    spec = importlib.util.spec_from_file_location("fig6", os.path.join(here, "validation", "piel2017_fig6.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rng = np.random.RandomState(5)
+    n = 2 ** 16
+    w = rng.normal(size=n)
+    assert abs(mod.blocking_error(w) / (1 / np.sqrt(n)) - 1) < 0.3
+    phi = 0.98
+    x = np.empty(n)
+    x[0] = rng.normal() / np.sqrt(1 - phi ** 2)
+    e = rng.normal(size=n)
+    for i in range(1, n):
+        x[i] = phi * x[i - 1] + e[i]
+    sigma = 1 / np.sqrt(1 - phi ** 2)
+    exact = sigma / np.sqrt(n) * np.sqrt((1 + phi) / (1 - phi))
+    assert abs(mod.blocking_error(x) / exact - 1) < 0.35
