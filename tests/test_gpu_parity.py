@@ -15,7 +15,10 @@ Tolerances (stated here, checked below):
     that force error; reinjected ions to FP64 rounding), |dv| <= 1e-4 max|v|,
     dust charges, collection and dust-dust forces rel <= 1e-5 of their scale,
     orbit sums and forces within the propagation of the per-ion position
-    bounds through the orbit term (conftest.check_records).
+    bounds through the orbit term (conftest.check_records).  These force
+    bounds hold at the test sizes; at 2^20 ions the position quantization
+    grows with density, and test_full_size_sym_forces_vs_fp64_gpu states its
+    own split of arithmetic and quantization errors.
   * Integer/index outputs (flags, reinjection faces, absorbed counts) are
     bit-exact.
 """
