@@ -29,6 +29,7 @@ extern "C" {
 #define OD_ABI_VERSION 2
 #define OD_MAX_DUST 8
 #define OD_MAX_TABLE 512
+#define OD_MAX_CELLS (1 << 26) /* cutoff mode: cells of the grid below */
 
 /* status codes */
 #define OD_OK 0

@@ -258,10 +258,12 @@ struct od_ctx {
   double* sc_tp = nullptr;
   float* sc_sp = nullptr;
   int *scan_tsum = nullptr, *scan_toff = nullptr;
-  int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0;
+  int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0, rs_passes = 2;
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
+  uint32_t *skey = nullptr, *sval = nullptr;  // sorted keys/values: the buffers the last radix pass wrote
   int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_start = nullptr,
-      *cell_chunks = nullptr, *item_off = nullptr, *items = nullptr, *n_items = nullptr;
+      *cell_chunks = nullptr, *item_off = nullptr, *n_items = nullptr;
+  int2* items = nullptr;
   void* spos = nullptr;
 };
 
@@ -486,11 +488,14 @@ static int sym_default_chunk(const std::vector<int>& be, int configured) {
 static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
   c->cut = 0;
   if (!(cfg->cutoff > 0.0)) return OD_OK;
-  if ((double)hp.nx * hp.ny * hp.nz > 65535.0)
-    return set_err(OD_ERR_ARG, "cutoff %g nm gives %d x %d x %d cells; at most 65535 are supported", cfg->cutoff,
-                   hp.nx, hp.ny, hp.nz);
+  if ((double)hp.nx * hp.ny * hp.nz > (double)OD_MAX_CELLS)
+    return set_err(OD_ERR_ARG, "cutoff %g nm gives %d x %d x %d cells; at most %d are supported", cfg->cutoff,
+                   hp.nx, hp.ny, hp.nz, OD_MAX_CELLS);
   c->cut = 1;
   c->n_cells = hp.n_cells;
+  // 8-bit digits over the keys 0 .. n_cells (n_cells marks ions left out)
+  c->rs_passes = 1;
+  while (c->rs_passes < 4 && ((uint64_t)c->n_cells >> (8 * c->rs_passes)) != 0) ++c->rs_passes;
   c->rs_nblk = (c->n_pad + RS_ITEMS - 1) / RS_ITEMS;
   c->max_items = c->n_pad / 64 + c->n_cells + 1;
   const size_t np = c->n_pad;
@@ -500,10 +505,12 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
       (rc = dalloc(&c->rs_off, (size_t)256 * c->rs_nblk * 4)) || (rc = dalloc(&c->rs_total, 4)) ||
       (rc = dalloc(&c->cell_start, (size_t)(c->n_cells + 1) * 4)) ||
       (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cells + 1) * 4)) ||
-      (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * 4)) ||
+      (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * sizeof(int2))) ||
       (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)) ||
       (rc = dalloc(&c->Gcut, 3 * np * 8)))
     return rc;
+  c->skey = c->ckey[c->rs_passes & 1];
+  c->sval = c->cval[c->rs_passes & 1];
   const size_t n_tiles = (std::max<size_t>((size_t)256 * c->rs_nblk, (size_t)c->n_cells + 1) + 1023) / 1024;
   if ((rc = dalloc(&c->scan_tsum, n_tiles * 4)) || (rc = dalloc(&c->scan_toff, n_tiles * 4))) return rc;
   if (c->prec == OD_PREC_FP32C &&
@@ -684,8 +691,8 @@ static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, cons
   using PV = typename Pos<T>::V;
   const int np = c->n_pad, nb = c->nb, nc = c->n_cells;
   hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, c->stream, c->dP, X, flags, c->ckey[0], c->cval[0]);
-  for (int pass = 0; pass < 2; ++pass) {
-    const int in = pass, out = 1 - pass, shift = 8 * pass;
+  for (int pass = 0; pass < c->rs_passes; ++pass) {
+    const int in = pass & 1, out = 1 - in, shift = 8 * pass;
     hipLaunchKernelGGL(k_rs_hist, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], np, shift, c->rs_nblk,
                        c->rs_hist);
     launch_scan(c, c->rs_hist, 256 * c->rs_nblk, c->rs_off, c->rs_total);
@@ -693,12 +700,12 @@ static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, cons
                        c->rs_nblk, c->rs_off, c->ckey[out], c->cval[out]);
   }
   const int gc = (nc + 256) / 256;
-  hipLaunchKernelGGL(k_cell_starts, dim3(np / 256 + 1), dim3(256), 0, c->stream, c->ckey[0], np, nc,
+  hipLaunchKernelGGL(k_cell_starts, dim3(np / 256 + 1), dim3(256), 0, c->stream, c->skey, np, nc,
                      c->cell_start);
   hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_start, nc, c->cell_chunks);
   launch_scan(c, c->cell_chunks, nc, c->item_off, c->n_items);
   hipLaunchKernelGGL(k_cell_items, dim3(gc), dim3(256), 0, c->stream, c->cell_chunks, c->item_off, nc, c->items);
-  hipLaunchKernelGGL(k_cell_gather<T>, dim3(nb), dim3(256), 0, c->stream, c->ckey[0], c->cval[0], np, nc,
+  hipLaunchKernelGGL(k_cell_gather<T>, dim3(nb), dim3(256), 0, c->stream, c->skey, c->sval, np, nc,
                      (const PV*)pos, (PV*)c->spos);
   return OD_OK;
 }
@@ -720,23 +727,23 @@ static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const ui
     const float4* sp4 = (const float4*)c->spos;
     if (sw) {
       hipLaunchKernelGGL((k_pair32_symcell<2>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
-                         c->ckey[0], c->cell_start, c->sc_tp, c->sc_sp);
+                         c->skey, c->cell_start, c->sc_tp, c->sc_sp);
       hipLaunchKernelGGL((k_symcell_self<1>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
                          c->sc_tp);
     } else {
       hipLaunchKernelGGL((k_pair32_symcell<1>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
-                         c->ckey[0], c->cell_start, c->sc_tp, c->sc_sp);
+                         c->skey, c->cell_start, c->sc_tp, c->sc_sp);
       hipLaunchKernelGGL((k_symcell_self<0>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
                          c->sc_tp);
     }
-    hipLaunchKernelGGL(k_symcell_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->ckey[0],
-                       c->cval[0], c->cell_start, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
+    hipLaunchKernelGGL(k_symcell_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->skey,
+                       c->sval, c->cell_start, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
   } else if (sw) {
     hipLaunchKernelGGL((k_pair_cell<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->cval[0], c->cell_start, c->items, c->n_items, F);
+                       c->sval, c->cell_start, c->items, c->n_items, F);
   } else {
     hipLaunchKernelGGL((k_pair_cell<T, 0>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->cval[0], c->cell_start, c->items, c->n_items, F);
+                       c->sval, c->cell_start, c->items, c->n_items, F);
   }
   if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
   return OD_OK;
@@ -753,10 +760,10 @@ static void launch_corr_cell(od_ctx* c, const typename Pos<T>::V* list, int add,
   using PV = typename Pos<T>::V;
   if (c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff)
     hipLaunchKernelGGL((k_corr_cell<T, 1>), dim3(c->nb), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->cval[0], c->cell_start, c->n_flag, list, add, out);
+                       c->sval, c->cell_start, c->n_flag, list, add, out);
   else
     hipLaunchKernelGGL((k_corr_cell<T, 0>), dim3(c->nb), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->cval[0], c->cell_start, c->n_flag, list, add, out);
+                       c->sval, c->cell_start, c->n_flag, list, add, out);
 }
 
 template <class T>
@@ -1005,8 +1012,8 @@ extern "C" int od_get_cells(od_ctx* c, int32_t* cell_of_ion, int32_t* sorted_ion
   if (!c->cut) return set_err(OD_ERR_STATE, "od_get_cells: context has no cutoff (no cell list)");
   HIPCHK(hipSetDevice(c->dev));
   std::vector<uint32_t> key(c->n_pad), val(c->n_pad);
-  HIPCHK(hipMemcpyAsync(key.data(), c->ckey[0], c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(val.data(), c->cval[0], c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(key.data(), c->skey, c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(val.data(), c->sval, c->n_pad * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   int64_t nb = 0;
   for (int p = 0; p < c->n_pad; ++p) {
@@ -1210,7 +1217,9 @@ extern "C" int od_abi_sizes(int64_t* config_size, int64_t* record_size) {
 
 extern "C" const char* od_last_error(void) { return g_err.c_str(); }
 
-extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950, ABI " "1"; }
+#define OD_STR_(x) #x
+#define OD_STR(x) OD_STR_(x)
+extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950, ABI " OD_STR(OD_ABI_VERSION); }
 
 #include "od_batch.inc"
 

@@ -31,10 +31,12 @@ T_I = 290.113
 N_INF = 3.57e14
 
 
-def single_dust(N=16384, n=2000, precision="fp32c", seed=1, charge="consistent", **kw):
-    pp = PlasmaParametersInSIUnitsMaxwell(T_I, T_E, N_INF, 1.0, M_AR)
+def single_dust(N=16384, n=2000, precision="fp32c", seed=1, charge="consistent", mach=1.0, R=5.0, H=10.0, **kw):
+    """The single-dust flow; ``mach``, ``R`` and ``H`` (in r_D_e) vary the
+    drift and the cylinder for tests (at M < 0.36 the bottom face reinjects)."""
+    pp = PlasmaParametersInSIUnitsMaxwell(T_I, T_E, N_INF, mach, M_AR)
     lam = pp.r_D_e
-    sp = SimulationParametersInSIUnits(5 * lam, 10 * lam, int(N), int(n), 0.01 / pp.w_p_i, pp)
+    sp = SimulationParametersInSIUnits(R * lam, H * lam, int(N), int(n), 0.01 / pp.w_p_i, pp)
     dust = [DustParticle(0.0, 0.0, 0.0, 0.05 * lam, charge, -2e4 * abs(pp.e))]
     return OpenDust(pp, sp, OutputParameters(10 ** 9, 10 ** 9), dust, "Maxwellian", precision=precision, seed=seed,
                     **kw)

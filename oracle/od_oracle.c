@@ -155,6 +155,29 @@ void orc_pair_forces(const od_config* c, int n, const double* x, double* f, int 
   }
 }
 
+/* Test support for the truncated law: for rows i in [i0, i1), the number of
+ * other ions j whose distance lies within rel * cutoff of the cutoff, i.e. the
+ * pairs whose inclusion a relative rounding of r by up to rel can flip.  The
+ * distance is computed as in orc_pair_forces. */
+void orc_cut_edge_count(const od_config* c, int n, const double* x, int i0, int i1, double rel, int* out,
+                        int nthreads) {
+  const double lo = c->cutoff * (1.0 - rel), hi = c->cutoff * (1.0 + rel);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+  for (int i = i0; i < i1; ++i) {
+    int m = 0;
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double dx = xi - x[3 * j], dy = yi - x[3 * j + 1], dz = zi - x[3 * j + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      m += (r >= lo && r <= hi) ? 1 : 0;
+    }
+    out[i - i0] = m;
+  }
+}
+
 /* dust Coulomb + trap + field force on one ion, q[] = current dust charges */
 void orc_ext_force(const od_config* c, const double* q, const double* xi, double* fo) {
   double f[3] = {0.0, 0.0, 0.0};

@@ -41,6 +41,8 @@ def load():
     lib.orc_table_eval.argtypes = [P(Table), ctypes.c_double]
     lib.orc_table_eval.restype = ctypes.c_double
     lib.orc_pair_forces.argtypes = [P(Config), ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_cut_edge_count.argtypes = [P(Config), ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       P(ctypes.c_int), ctypes.c_int]
     lib.orc_ext_force.argtypes = [P(Config), dp, dp, dp]
     lib.orc_reinject.argtypes = [P(Config), ctypes.c_uint32, ctypes.c_uint32, dp, dp]
     lib.orc_reinject.restype = ctypes.c_int
@@ -96,6 +98,30 @@ def pair_forces(cfg, x, rows=None, nthreads=0):
     f = np.empty((i1 - i0, 3))
     lib.orc_pair_forces(ctypes.byref(cfg), n, _d(x), _d(f), i0, i1, nthreads or threads())
     return f
+
+
+def cut_edge_count(cfg, x, rel, rows=None, nthreads=0):
+    """Per row: other ions at a distance within rel * cutoff of the cutoff
+    (pairs whose truncation a relative rounding of r by rel may flip)."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    i0, i1 = (0, n) if rows is None else rows
+    out = np.empty(i1 - i0, dtype=np.int32)
+    lib.orc_cut_edge_count(ctypes.byref(cfg), n, _d(x), i0, i1, float(rel),
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), nthreads or threads())
+    return out
+
+
+def cut_force_at_cutoff(cfg):
+    """Magnitude of one pair's force just inside the cutoff (kJ/mol/nm): the
+    most a single flipped truncation decision can change a row, with the
+    switch: zero at the cutoff."""
+    if cfg.switch_dist > 0.0 and cfg.switch_dist < cfg.cutoff:
+        return 0.0
+    r, lam = cfg.cutoff, cfg.r_D_e
+    rr = r + cfg.r_0
+    return cfg.kqq * np.exp(-r / lam) * (1.0 / (lam * rr) + 1.0 / (rr * rr))
 
 
 def ext_force(cfg, q, xi):
@@ -239,6 +265,34 @@ def fp32c_position_bounds(oracle_mod, cfg, x0, flags_orc):
     rms = np.sqrt(np.mean(fn ** 2))
     kept = np.asarray(flags_orc) == 0
     return np.where(kept, cfg.dt * 0.5 * cfg.dt / cfg.m_s * (1e-4 * fn + 1e-5 * rms), 0.0) + 1e-12 * cfg.R
+
+
+def fp32c_rounded_positions(cfg, x):
+    """The positions the FP32 pair kernels see: x scaled to units of
+    r_D_e / log2(e), rounded to FP32 (nearest even), scaled back in FP64."""
+    s = 1.4426950408889634 / cfg.r_D_e
+    return (np.asarray(x, dtype=np.float64) * s).astype(np.float32).astype(np.float64) / s
+
+
+def fp32c_position_bounds_measured(oracle_mod, cfg, x0, flags_orc):
+    """Per-ion bound of |x_gpu - x_orc| after one FP32c step from x0 at any
+    density.  The FP32c force error splits into the rounding of the positions
+    (measured here exactly: FP64 forces on the rounded positions minus FP64
+    forces on x0) and the kernel's arithmetic on the rounded positions
+    (|dF_i| <= 1e-5 |F_i| + 1e-6 rms|F|, compensated sums); a kept ion drifts
+    by dt (dt/2) / m_s times that force error."""
+    f = oracle_mod.pair_forces(cfg, x0)
+    xr = fp32c_rounded_positions(cfg, x0)
+    q = np.linalg.norm(oracle_mod.pair_forces(cfg, xr) - f, axis=1)
+    fn = np.linalg.norm(f, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    df = q + 1e-5 * fn + 1e-6 * rms
+    if cfg.cutoff > 0.0:
+        # a pair within the FP32 rounding of r of a hard cutoff may be counted
+        # on one side only: one pair force at the cutoff each
+        df = df + cut_edge_count(cfg, xr, 1e-6) * cut_force_at_cutoff(cfg) * 1.001
+    kept = np.asarray(flags_orc) == 0
+    return np.where(kept, cfg.dt * 0.5 * cfg.dt / cfg.m_s * df, 0.0) + 1e-12 * cfg.R
 
 
 def fp32c_position_error_ok(oracle_mod, cfg, x0, x_gpu, x_orc, flags_orc):

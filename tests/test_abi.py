@@ -96,7 +96,9 @@ def test_engine_host_sampler_matches_oracle(eng):
 
 
 def test_build_info(eng):
-    assert b"gfx950" in eng.load_library().od_build_info()
+    info = eng.load_library().od_build_info()
+    assert b"gfx950" in info
+    assert info.endswith(b"ABI %d" % eng.ABI_VERSION)
 
 
 def test_wrappers_are_context_managers(eng):

@@ -344,7 +344,7 @@ def test_tune_is_reproducible():
     assert np.array_equal(outs[0][1][0], outs[1][1][0]) and np.array_equal(outs[0][1][1], outs[1][1][1])
 
 
-CUT_CASES = [("fp64", 1.5, None), ("fp32c", 1.5, None), ("fp32c", 3.2, 3.0)]
+CUT_CASES = [("fp64", 1.5, None), ("fp32c", 1.5, None), ("fp32c", 3.2, 3.0), ("fp64", 3.2, 3.0)]
 
 
 @pytest.mark.parametrize("precision,rc,rs", CUT_CASES)
@@ -418,6 +418,69 @@ def test_cutoff_one_step_and_determinism(oracle_lib, precision):
         e.close()
     for r, x in outs[1:]:
         assert np.array_equal(r, outs[0][0]) and np.array_equal(x, outs[0][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+@pytest.mark.parametrize("thresholds", ["reference", "bottom_heavy"])
+def test_bottom_face_reinjection_vs_oracle(oracle_lib, precision, thresholds):
+    """Bottom-face reinjection (opendust.py:939-951), live whenever p_b != 0:
+    Maxwellian flow at M = 0.1 (p_b = 0.030), 400 ions crafted beyond the end
+    caps and the side wall.  "reference": the reference's face probabilities;
+    "bottom_heavy": thresholds moved to top 0.3, side 0.3, bottom 0.4, so that
+    the bottom branch runs for many ions.  Faces and flags bit-exact; bottom
+    ions exactly at z = +H/2 with v_z <= 0; positions and velocities of every
+    reinjected ion to FP64 rounding (1e-12 of R and of max|v|), other ions as
+    the one-step tests state them."""
+    from opendust_amd import host, workloads
+
+    eng = _engine_mod()
+    with contextlib.redirect_stdout(io.StringIO()):
+        od = workloads.single_dust(N=2048, n=3, precision=precision, seed=31, mach=0.1)
+        cfg, x0, v0 = workloads.prepare(od, alpha=workloads.synthetic_alpha(od), np_seed=4, precision=precision)
+    assert cfg.use_bottom == 1 and cfg.bottom.n_seg > 0
+    if thresholds == "bottom_heavy":
+        cfg.thr_top = host.u32_threshold(0.3)
+        cfg.thr_side = host.u32_threshold(0.6)
+    rng = np.random.RandomState(17)
+    for i in range(200):
+        x0[i, 2] = (0.5 + rng.uniform(0.002, 0.02)) * cfg.H * (1 if i % 2 else -1)
+    for i in range(200, 400):
+        a = rng.uniform(0, 2 * np.pi)
+        x0[i] = [1.02 * cfg.R * np.cos(a), 1.02 * cfg.R * np.sin(a), rng.uniform(-0.45, 0.45) * cfg.H]
+    q0 = np.array([cfg.dust_q0[0]])
+    say("bottom face", precision, thresholds, "N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    e.set_ions(x0, v0)
+    e.set_step(4)
+    rec = e.run(1)
+    x_g, v_g = e.get_ions()
+    fl_g, face_g, nre = e.step_info()
+    e.close()
+    x_o, v_o, q_o, rec_o, fl_o, face_o = oracle_lib.step(cfg, x0, v0, q0, 4)
+    counts = np.bincount(face_o, minlength=4)
+    say("  faces [none, top, side, bottom]", counts.tolist())
+    assert np.array_equal(fl_g, fl_o)
+    assert np.array_equal(face_g, face_o)
+    assert counts[3] >= (5 if thresholds == "reference" else 120)
+    b = face_g == 3
+    assert np.all(x_g[b, 2] == cfg.H / 2) and np.all(v_g[b, 2] <= 0.0)
+    re = face_o != 0
+    dx_re = np.max(np.abs(x_g[re] - x_o[re])) / cfg.R
+    dv_re = np.max(np.abs(v_g[re] - v_o[re])) / np.max(np.abs(v_o))
+    say("  reinjected ions: max |dx|/R", dx_re, "max |dv|/max|v|", dv_re)
+    assert dx_re <= 1e-12 and dv_re <= 1e-12
+    assert np.array_equal(rec["absorbed"][0], rec_o["absorbed"])
+    bound = None
+    if precision == "fp64":
+        assert np.max(np.abs(x_g - x_o)) <= 1e-11 * cfg.R
+    else:
+        ok, worst, bound = fp32c_position_error_ok(oracle_lib, cfg, x0, x_g, x_o, fl_o)
+        say("  worst |dx_i| / bound_i", worst)
+        assert ok
+    ratios = check_records(cfg, rec[0], rec_o, precision, x_o, bound)
+    say("  records, error / tolerance:", ratios)
+    for key, r in ratios.items():
+        assert r <= 1.0, key
 
 
 def test_trap_grid_vs_oracle(oracle_lib):

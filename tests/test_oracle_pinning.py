@@ -150,6 +150,24 @@ def test_binning_spec(orc, base):
         cfg.cutoff = 0.0
 
 
+def test_binning_spec_beyond_16_bit_cell_ids(orc, base):
+    """A grid of more than 65535 cells (R = 10, H = 40 r_D_e at a 1 r_D_e
+    cutoff: 40 x 40 x 80 cells), where cell ids need a third 8-bit radix
+    digit: the same formula, no wrap-around."""
+    _, cfg, _, _ = base
+    save = (cfg.R, cfg.H, cfg.cutoff)
+    lam = cfg.r_D_e
+    try:
+        cfg.R, cfg.H, cfg.cutoff = 10 * lam, 40 * lam, 1.0 * lam
+        h = cfg.cutoff / 2
+        cid, dims = orc.cell_of(cfg, np.array([cfg.R, cfg.R, cfg.H / 2]))
+        assert dims == (40, 40, 80) and cid == 40 * 40 * 80 - 1 > 65535
+        p = np.array([-cfg.R + 17.5 * h, -cfg.R + 33.5 * h, -cfg.H / 2 + 71.5 * h])
+        assert orc.cell_of(cfg, p)[0] == 17 + 40 * (33 + 40 * 71)
+    finally:
+        cfg.R, cfg.H, cfg.cutoff = save
+
+
 def _ext_energy(cfg, q, x):
     e = 0.0
     for p in range(cfg.n_dust):
@@ -365,6 +383,120 @@ def test_reinjection_distributions(orc, base):
     assert stats.kstest(v_in, cdf_s).statistic <= 1.63 / np.sqrt(s.sum()) + 3e-3
     assert stats.kstest(v_tan / cfg.v_T, "norm").pvalue > 1e-4
     assert stats.kstest((vs[s, 2] - cfg.v_fl) / cfg.v_T, "norm").pvalue > 1e-4
+
+
+def _flux(u, v_T, a, b):
+    """Flux of a Maxwellian with drift u (toward the face) through the face,
+    per unit density, over normal speeds [a, b]: int_a^b v N(v; u, v_T) dv,
+    from the antiderivative -v_T^2 N(v; u, v_T) + u Phi((v - u) / v_T)."""
+    s2 = np.sqrt(2) * v_T
+    G = lambda v: -v_T / np.sqrt(2 * np.pi) * np.exp(-((v - u) / s2) ** 2) + u / 2 * special.erf((v - u) / s2)
+    return G(b) - G(a)
+
+
+@pytest.mark.parametrize("mach", [0.1, 0.3, 1.0])
+def test_face_probabilities_are_closed_form_fluxes(mach):
+    """p_t, p_s, p_b (opendust.py:806-837) are the faces' shares of the
+    inflowing flux: top pi R^2 G(v_fl), bottom pi R^2 G(-v_fl), side 2 pi R H
+    G(0), with G the closed-form flux of the drifting Maxwellian over the
+    speed ranges the reference integrates (opendust.py:608-632): top
+    [max(v_fl - 4 v_T, 0), v_fl + 4 v_T], bottom [0, 4 v_T - v_fl] (empty,
+    p_b = 0, once that is below 0.4 v_T, i.e. M >= 0.36 here), side [0, 4 v_T]."""
+    import contextlib
+    import io
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        od = workloads.single_dust(N=64, n=3, mach=mach)
+        workloads.prepare(od, alpha=np.zeros(15), np_seed=1)
+    v_T, v_fl = od.v_T, od.v_fl
+    G_t = _flux(v_fl, v_T, max(v_fl - 4 * v_T, 0), v_fl + 4 * v_T)
+    G_s = _flux(0.0, v_T, 0.0, 4 * v_T)
+    hi_b = max(-v_fl + 4 * v_T, 0)
+    G_b = _flux(-v_fl, v_T, 0.0, hi_b) if hi_b >= 0.4 * v_T else 0.0
+    area_t, area_s = np.pi * od.R ** 2, 2 * np.pi * od.R * od.H
+    tot = area_t * (G_t + G_b) + area_s * G_s
+    assert od.p_t == pytest.approx(area_t * G_t / tot, rel=1e-7)
+    assert od.p_s == pytest.approx(area_s * G_s / tot, rel=1e-7)
+    if mach >= 0.36:
+        assert od.p_b == 0.0
+    else:
+        assert od.p_b > 0.0
+        assert od.p_b == pytest.approx(area_t * G_b / tot, rel=1e-6)
+
+
+def test_bottom_face_reinjection_distributions(orc):
+    """Bottom face (opendust.py:939-951, used when p_b != 0): at M = 0.1 the
+    reference's p_b = 0.030.  With the real thresholds the bottom share
+    matches p_b; with thresholds moved so that 60 % of the draws choose the
+    bottom, the bottom ions sit at z = +H/2 exactly, uniform over the disc,
+    v_x and v_y ~ N(0, v_T), and -v_z follows the flux-weighted distribution
+    of the drift -v_fl on [0, max(4 v_T - v_fl, 0)] (the table built with
+    -v_fl, opendust.py:625-632)."""
+    import contextlib
+    import io
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        od = workloads.single_dust(N=64, n=3, mach=0.1, seed=23)
+        cfg, _, _ = workloads.prepare(od, alpha=np.zeros(15), np_seed=1)
+    assert cfg.use_bottom == 1 and cfg.bottom.n_seg > 0
+    n = 40000
+    faces, _, _ = _reinject_many(orc, cfg, n)
+    p_b = 1.0 - cfg.thr_side / 2 ** 32
+    assert p_b == pytest.approx(od.p_b, abs=1e-9)
+    assert abs(np.mean(faces == 3) - p_b) <= 5 * np.sqrt(p_b * (1 - p_b) / n)
+    cfg.thr_top = host.u32_threshold(0.2)
+    cfg.thr_side = host.u32_threshold(0.4)
+    faces, xs, vs = _reinject_many(orc, cfg, n)
+    b = faces == 3
+    assert abs(np.mean(b) - 0.6) <= 5 * np.sqrt(0.24 / n)
+    assert abs(np.mean(faces == 1) - 0.2) <= 5 * np.sqrt(0.16 / n)
+    assert np.all(xs[b, 2] == cfg.H / 2)
+    assert np.all(xs[faces == 1, 2] == -cfg.H / 2)
+    assert stats.kstest((xs[b, 0] ** 2 + xs[b, 1] ** 2) / cfg.R ** 2, "uniform").pvalue > 1e-4
+    assert stats.kstest(vs[b, 0] / cfg.v_T, "norm").pvalue > 1e-4
+    assert stats.kstest(vs[b, 1] / cfg.v_T, "norm").pvalue > 1e-4
+    assert np.all(vs[b, 2] <= 0.0)
+    s2 = np.sqrt(2) * cfg.v_T
+    vf = -cfg.v_fl
+    hi = max(vf + 4 * cfg.v_T, 0.0)
+
+    def G(v):
+        return cfg.v_T / np.sqrt(2 * np.pi) * (-np.exp(-((v - vf) / s2) ** 2)) + vf / 2 * special.erf((v - vf) / s2)
+
+    cdf = lambda w: (G(np.clip(w, 0, hi)) - G(0.0)) / (G(hi) - G(0.0))
+    # the table's knots are the closed-form CDF of that flux; between knots the
+    # sampled speed is linear in u, so its CDF is the linear interpolant of
+    # the knots, which the sample must follow to the KS bound
+    _, tab = host.face_table(vf, cfg.v_T, "bottom", "Maxwellian")
+    knots = -tab.grid_v
+    assert np.all(np.diff(knots) > 0) and knots[0] == 0.0 and knots[-1] == pytest.approx(hi, rel=1e-12)
+    assert np.max(np.abs(tab.grid_xi - cdf(knots))) <= 1e-7
+    D = stats.kstest(-vs[b, 2], lambda w: np.interp(w, knots, cdf(knots))).statistic
+    assert D <= 1.63 / np.sqrt(b.sum())
+    # a sign slip (the top table, +v_fl) is far outside that bound
+    vf_wrong = cfg.v_fl
+    G_w = lambda v: (cfg.v_T / np.sqrt(2 * np.pi) * (-np.exp(-((v - vf_wrong) / s2) ** 2))
+                     + vf_wrong / 2 * special.erf((v - vf_wrong) / s2))
+    hi_w = vf_wrong + 4 * cfg.v_T
+    cdf_w = lambda w: (G_w(np.clip(w, 0, hi_w)) - G_w(0.0)) / (G_w(hi_w) - G_w(0.0))
+    assert stats.kstest(-vs[b, 2], cdf_w).statistic > 10 * (1.63 / np.sqrt(b.sum()))
+
+
+def test_cut_edge_count_matches_brute_force(orc):
+    """The test helper counting pairs near the cutoff, against NumPy."""
+    import contextlib
+    import io
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        od = workloads.single_dust(N=400, n=3, R=1.0, H=2.0)
+        cfg, x0, _ = workloads.prepare(od, alpha=np.zeros(15), np_seed=4)
+    cfg.cutoff = 0.7 * cfg.r_D_e
+    d = np.linalg.norm(x0[:, None, :] - x0[None, :, :], axis=2)
+    np.fill_diagonal(d, -1.0)
+    for rel in (1e-2, 1e-3):
+        ref = np.sum(np.abs(d - cfg.cutoff) <= rel * cfg.cutoff, axis=1)
+        assert np.array_equal(orc.cut_edge_count(cfg, x0, rel, nthreads=2), ref)
+    assert orc.cut_edge_count(cfg, x0, 1e-2).sum() > 0
 
 
 def test_collisions_rate_and_distribution(orc):
