@@ -272,6 +272,15 @@ int od_trap_samples_host(double R, double H, uint64_t seed, int64_t k0, int64_t 
  * the tool's sequential sums in ion order, bit for bit. */
 int od_density_profile(od_ctx* ctx, int32_t n_nodes, double R, double L, double N_s, double* z_grid, double* conc);
 
+/* od_density_profile without any device allocation, for callers that cannot
+ * allocate (tight dust-dynamics or serving loops, code that must not call
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results as
+ * od_density_profile (bit-identical); n_nodes is at most 65536, the size of
+ * the library's per-device scratch array (1 MiB, held until the process ends).
+ * Calls that overlap on one GPU share that array and race. */
+int od_density_profile_noalloc(od_ctx* ctx, int32_t n_nodes, double R, double L, double N_s, double* z_grid,
+                               double* conc);
+
 /* The same for host positions z_nm[N] (nm), e.g. one frame of an XYZ file. */
 int od_density_profile_z(int32_t device, const double* z_nm, int64_t N, int32_t n_nodes, double R, double L,
                          double N_s, double* z_grid, double* conc);

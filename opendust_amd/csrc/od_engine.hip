@@ -26,6 +26,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -59,6 +60,10 @@ int od_set_error_message(int code, const char* msg) { return set_err(code, "%s",
     if (e_ != hipSuccess)                                                                              \
       return set_err(OD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
   } while (0)
+
+// per-device address of the allocation-free entry points' scratch (defined
+// with od_density_profile_noalloc below)
+static int scratch_address(int dev, double** out);
 
 #define CHKRET(expr)      \
   do {                    \
@@ -592,6 +597,11 @@ extern "C" int od_create(const od_config* cfg, od_ctx** out) {
   if (hipSetDevice(c->dev) != hipSuccess) return fail(set_err(OD_ERR_HIP, "hipSetDevice(%d) failed", c->dev));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_err(OD_ERR_HIP, "hipStreamCreate failed"));
+  {
+    // per-device address of the allocation-free entry points' scratch array
+    double* sp = nullptr;
+    if ((rc = scratch_address(c->dev, &sp))) return fail(rc);
+  }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return fail(set_err(OD_ERR_HIP, "hipEventCreate failed"));
   const size_t np = c->n_pad;
@@ -1424,6 +1434,63 @@ extern "C" int od_density_profile(od_ctx* c, int32_t n_nodes, double R, double L
   if (!c->have_ions) return set_err(OD_ERR_STATE, "density profile: no ions yet (call od_set_ions first)");
   HIPCHK(hipSetDevice(c->dev));
   return density_run(c->stream, c->X + 2 * (size_t)c->n_pad, c->n, n_nodes, R, L, N_s, z_grid, conc);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free density profile.  Its scratch (the node positions and the
+// per-node sums k_density writes) lives in one file-scope device array; each
+// GPU holds its own copy, whose address is looked up once per device (at
+// od_create, or at the first call on a device) and kept in g_scratch_ptr.
+// Host code uses only that address, never the array's name.  Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU race.  Nothing
+// needs clearing: each call uploads all node positions before the kernel and
+// the kernel writes every node's sum.
+// ---------------------------------------------------------------------------
+constexpr int OD_NOALLOC_MAX_NODES = 65536;
+static __device__ __align__(256) double scratch[2 * OD_NOALLOC_MAX_NODES];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device; null until looked up
+
+// address of this device's copy of `scratch`; the device must be current
+static int scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(OD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return OD_OK;
+}
+
+extern "C" int od_density_profile_noalloc(od_ctx* c, int32_t n_nodes, double R, double L, double N_s,
+                                          double* z_grid, double* conc) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (n_nodes > OD_NOALLOC_MAX_NODES)
+    return set_err(OD_ERR_ARG, "od_density_profile_noalloc: %d nodes exceed the scratch maximum of %d", n_nodes,
+                   OD_NOALLOC_MAX_NODES);
+  CHKRET(density_check(n_nodes, R, L, N_s));
+  if (!c->have_ions) return set_err(OD_ERR_STATE, "density profile: no ions yet (call od_set_ions first)");
+  HIPCHK(hipSetDevice(c->dev));
+  double* sp = nullptr;
+  CHKRET(scratch_address(c->dev, &sp));
+  const int n = n_nodes;
+  std::vector<double> zg(n), sums(n);
+  for (int i = 0; i < n; i++) zg[i] = -L / 2.0 + i * L / (double)(n - 1);
+  double* d_zg = sp;
+  double* d_s = sp + OD_NOALLOC_MAX_NODES;
+  HIPCHK(hipMemcpyAsync(d_zg, zg.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_density, dim3(n), dim3(64), 0, c->stream, (const double*)(c->X + 2 * (size_t)c->n_pad),
+                     (int64_t)c->n, n, L, (const double*)d_zg, d_s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(sums.data(), d_s, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  density_finish(n, R, L, N_s, sums, z_grid, conc);
+  return OD_OK;
 }
 
 extern "C" int od_density_profile_z(int32_t device, const double* z_nm, int64_t N, int32_t n_nodes, double R,

@@ -147,7 +147,7 @@ EXPORTS = [
     "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune", "od_sym_variants", "od_sym_variant_info",
     "od_sym_choice",
     "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
-    "od_update_config",
+    "od_update_config", "od_density_profile_noalloc",
 ]
 
 
@@ -200,6 +200,8 @@ def load_library(path=LIB_PATH):
     lib.od_write_restart.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int64]
     lib.od_density_profile.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        dp, dp]
+    lib.od_density_profile_noalloc.argtypes = [c_ctx, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, dp, dp]
     lib.od_density_profile_z.argtypes = [ctypes.c_int32, dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, dp, dp]
     lib.od_batch_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
@@ -416,6 +418,15 @@ class Engine:
         c = np.empty(n)
         _check(self.lib.od_density_profile(self._ctx, int(n), float(R), float(L), float(N_s), _dptr(zg), _dptr(c)),
                "od_density_profile")
+        return zg, c
+
+    def density_profile_noalloc(self, R, L, N_s, n=100):
+        """density_profile through the entry point that allocates nothing on
+        the device (its scratch is a fixed per-device array; n <= 65536)."""
+        zg = np.empty(n)
+        c = np.empty(n)
+        _check(self.lib.od_density_profile_noalloc(self._ctx, int(n), float(R), float(L), float(N_s), _dptr(zg),
+                                                   _dptr(c)), "od_density_profile_noalloc")
         return zg, c
 
     def tune(self, reps=3):
