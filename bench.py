@@ -41,6 +41,38 @@ SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes pe
     (4, 1): 31.020e9, (4, 2): 22.758e9, (4, 4): 18.547e9, (4, 8): 16.588e9, (4, 16): 15.480e9, (4, 32): 14.710e9,
 }
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+# Algorithmic HBM bytes per ion of the per-ion stages of a step (FP32c): what
+# the stage must read and write once, flagged-ion lists excluded.
+STAGE_BYTES_PER_ION = {
+    "kick_drift": 156,  # read X, V, F (72); write X, V (48), flags (4), FP32 positions pi, pj (32)
+    "list": 4,          # read the flags
+    "kick2": 117,       # read pi (16), flags (4), X, V, F (72); write V (24), face (1)
+    "inflow": 68,       # read pj (16), flags (4); read and write F (48)
+}
+
+
+def cell_stage_bytes_per_ion(n_cells):
+    """Binning (read X 24 + flags 4, write key and value 8), one 8-bit radix
+    pass per digit of the cell ids (read key 4 for the histogram, read and
+    write key and value 16), gather (read key, value 8 and the position 16,
+    write the sorted position 16)."""
+    passes = 1
+    while passes < 4 and (n_cells >> (8 * passes)) != 0:
+        passes += 1
+    return 36 + 20 * passes + 40, passes
+
+
+def hbm_stages(stage_ms, n_ions, bytes_per_ion):
+    """GB/s of the per-ion stages against the HBM peak."""
+    out = {}
+    for name, b in bytes_per_ion.items():
+        ms = stage_ms.get(name, 0.0)
+        if ms <= 0:
+            continue
+        gbs = b * n_ions / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "bytes_per_ion": b, "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def log(*a):
@@ -60,10 +92,16 @@ def copy_calibration(nbytes=1 << 30, reps=8):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
-def cpu_baseline(cfg, x, target_s=12.0):
-    """FP64 oracle pair-force rows against all N ions, all host threads."""
+def cpu_baseline(cfg, x, target_s=12.0, small_steps=10):
+    """The FP64 oracle on the host CPU, rebuilt there (-O3 -march=native), all
+    host threads: (a) ion-ion force rows of a sample of the benchmark's 2^20
+    ions against all of them (the workload of `value`), (b) full oracle steps
+    of the 16384-ion single-dust workload (two O(N^2) force evaluations per
+    step, as the reference's step)."""
     from oracle import oracle
+    from opendust_amd import workloads
 
+    build = oracle.use_native_build()
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.threads()
     n = x.shape[0]
     rows = 8 * nthreads
@@ -78,6 +116,17 @@ def cpu_baseline(cfg, x, target_s=12.0):
     oracle.pair_forces(cfg, x, rows=(0, rows), nthreads=nthreads)
     dt = time.perf_counter() - t0
     pairs = rows * (n - 1)
+    import contextlib
+    import io
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        od0 = workloads.single_dust(N=16384, n=small_steps, precision="fp64", seed=12345)
+        cfg0, x0, v0 = workloads.prepare(od0, alpha=workloads.synthetic_alpha(od0), np_seed=7, precision="fp64")
+    q0 = np.array([cfg0.dust_q0[0]])
+    t0 = time.perf_counter()
+    oracle.run(cfg0, x0, v0, q0, 0, small_steps, nthreads=nthreads)
+    dt0 = time.perf_counter() - t0
+    n0 = cfg0.n_ions
     return {
         "value": pairs / dt,
         "unit": "pair interactions/s",
@@ -85,6 +134,17 @@ def cpu_baseline(cfg, x, target_s=12.0):
         "kind": "port",
         "sample": "FP64 oracle ion-ion force rows 0..%d of %d ions against all %d ions (%.1f s); "
                   "the oracle step evaluates this O(N^2) force twice per step" % (rows, n, n, dt),
+        "cpu_model": oracle.cpu_model(),
+        "os_cpu_count": os.cpu_count(),
+        "threads": nthreads,
+        "build": build,
+        "small_workload_full_steps": {
+            "workload": "single dust, 16384 superions, R=5 r_D_e, H=10 r_D_e, FP64, full oracle steps",
+            "steps": small_steps,
+            "s_per_step": dt0 / small_steps,
+            "steps_per_s": small_steps / dt0,
+            "pair_interactions_per_s": float(n0) * (n0 - 1) * small_steps / dt0,
+        },
     }
 
 
@@ -97,6 +157,7 @@ def cutoff_leg(steps):
     cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=11, precision="fp32c")
     e = engine.Engine(cfg)
     e.set_ions(x0, v0)
+    _, variant = e.tune(2)
     e.run(1, engine.RUN_PROFILE_PAIR)
     e.pair_timing(reset=True)
     t0 = time.perf_counter()
@@ -104,7 +165,12 @@ def cutoff_leg(steps):
     wall = time.perf_counter() - t0
     ms, nl = e.pair_timing()
     in_range, cand = e.cutoff_pair_count()
+    _, _, dims = e.cells()
+    e.run(2, engine.RUN_PROFILE_STAGES)
+    st, _ = e.stage_timing()
     e.close()
+    n_cells = dims[0] * dims[1] * dims[2]
+    cb, passes = cell_stage_bytes_per_ion(n_cells)
     return {
         "workload": "single dust, 2^22 superions, R=10 r_D_e, H=20 r_D_e, hard cutoff 2 r_D_e, cell list rebuilt "
                     "every step",
@@ -115,6 +181,12 @@ def cutoff_leg(steps):
         "pair_interactions_per_s": in_range * steps / wall,
         "cell_kernel_ms": ms / max(nl, 1),
         "cell_kernel_tflops_alg": in_range * FLOP_PER_PAIR / (ms / max(nl, 1) * 1e-3) / 1e12,
+        "cell_kernel_frac_of_fp32_peak": in_range * FLOP_PER_PAIR / (ms / max(nl, 1) * 1e-3) / 1e12
+                                         / FP32_PEAK_TFLOPS,
+        "cell_kernel_variant": {"sym_tile": variant, "targets_per_lane": {1: 1, 2: 2, 3: 4}.get(variant)},
+        "grid": {"dims": list(dims), "cells": n_cells, "radix_passes": passes},
+        "stage_ms": st,
+        "hbm_stages": hbm_stages(st, 2 ** 22, dict(STAGE_BYTES_PER_ION, cells=cb)),
     }
 
 
@@ -182,6 +254,61 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
     }
 
 
+def sweep_full(n_sims=64, sweep_len=5000, seq_steps=200, out_path=None):
+    """The whole parameter sweep end to end, setup included: batched, every
+    simulation's sweep_len steps (simulateBatch: trap fits, initial states,
+    tables, batch, graph-replayed steps, records); sequential, the same
+    simulations one after another (simulate-like: trap fit, initial state,
+    context, graph-replayed steps, records), each timed over seq_steps steps
+    and its step loop extrapolated to sweep_len (its setup measured)."""
+    import contextlib
+    import io
+
+    from opendust_amd import engine, sweep, workloads
+
+    def make():
+        with contextlib.redirect_stdout(io.StringIO()):
+            return workloads.sweep(n_sims=n_sims, n=sweep_len, precision="fp32c", seed=7, trapSeed=5,
+                                   trapSamples=4_000_000)
+
+    ods = make()
+    np.random.seed(21)
+    log("sweep end to end: batched, %d steps" % sweep_len)
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(io.StringIO()):
+        sweep.simulateBatch(ods, "0", useGraph=True)
+    t_batch = time.perf_counter() - t0
+    log("sweep end to end: batched %.1f s" % t_batch)
+    ods = make()
+    np.random.seed(21)
+    t_seq, t_setup = 0.0, 0.0
+    for i, od in enumerate(ods):
+        t0 = time.perf_counter()
+        cfg, x0, v0 = workloads.prepare(od, alpha=None, np_seed=None, precision="fp32c")
+        e = engine.Engine(cfg)
+        e.set_ions(x0, v0)
+        t1 = time.perf_counter()
+        e.run(seq_steps, engine.RUN_GRAPH)
+        t2 = time.perf_counter()
+        e.close()
+        t_setup += t1 - t0
+        t_seq += (t1 - t0) + (t2 - t1) * sweep_len / seq_steps
+        if i % 16 == 15:
+            log("sweep end to end: sequential, %d of %d simulations" % (i + 1, n_sims))
+    res = {
+        "workload": "%d simulations of the sweep generator (seed 7), %d steps each, FP32c" % (n_sims, sweep_len),
+        "batched_wall_s": t_batch,
+        "sequential_wall_s_extrapolated": t_seq,
+        "sequential_setup_s": t_setup,
+        "sequential_steps_timed_per_sim": seq_steps,
+        "sims_per_hour_batched": n_sims / t_batch * 3600.0,
+        "sims_per_hour_sequential": n_sims / t_seq * 3600.0,
+        "batched_over_sequential": t_seq / t_batch,
+    }
+    print(json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +332,8 @@ def main():
                     help="symmetric kernel variant (1-based; tile shape and waves per SIMD); 0: autotune, or "
                          "the size-dependent default with --no-tune or --sym-chunk")
     ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
+    ap.add_argument("--sweep-full", action="store_true",
+                    help="only: the whole 64-simulation, 5000-step sweep end to end, batched and sequential")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -213,6 +342,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.sweep_full:
+        sweep_full()
         return
 
     from opendust_amd import engine, workloads
@@ -254,7 +386,12 @@ def main():
     pair_ms, launches = eng.pair_timing()
     x_mid, v_mid = eng.get_ions()
     q_mid = eng.get_dust_charges()
+    # per-stage HIP-event times of a few more steps (outside the timed region)
+    eng.run(3, engine.RUN_PROFILE_STAGES)
+    stage_ms, _ = eng.stage_timing()
+    n_flagged = eng.step_info()[2]
     eng.close()
+    log("stages (ms/step):", {k: round(v, 4) for k, v in stage_ms.items()})
 
     ms_per_step = wall / args.steps * 1e3
     pairs_step = float(N) * (N - 1)
@@ -350,13 +487,28 @@ def main():
             "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd"), engine.sym_variants()[tile - 1]))
                                if tile else None),
         },
+        "stage_ms": stage_ms,
+        "hbm_stages": hbm_stages(stage_ms, N, STAGE_BYTES_PER_ION),
+        "flagged_ions_last_step": n_flagged,
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
         "cutoff_2e22": cut,
         "sweep_64": swp,
         "absorbed_last_step": int(rec["absorbed"][-1, 0]),
     }
+    # targets of the workload (the dust-force accuracy is a correctness
+    # requirement: the run fails when it is missed)
+    dust_err = fp64["dust_force_rel_err_first_step"] if fp64 else None
+    out["targets"] = {
+        "fp32c_pair_kernel_frac_ge_0.60": out["roofline"]["frac"] >= 0.60,
+        "fp32c_ge_5x_fp64": (fp64["fp32c_speedup"] >= 5.0) if fp64 else None,
+        "dust_force_rel_err_le_1e-5": (dust_err <= 1e-5) if dust_err is not None else None,
+        "sweep_ge_3x_sequential": (swp["batched_over_sequential"] >= 3.0) if swp else None,
+    }
     print(json.dumps(out))
+    if dust_err is not None and not dust_err <= 1e-5:
+        log("dust force error %.3g against the FP64 path exceeds the 1e-5 target" % dust_err)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -29,7 +29,7 @@ extern "C" {
 #define OD_ABI_VERSION 2
 #define OD_MAX_DUST 8
 #define OD_MAX_TABLE 512
-#define OD_MAX_CELLS (1 << 26) /* cutoff mode: cells of the grid below */
+#define OD_MAX_CELLS (1 << 27) /* cutoff mode: cells of the grid below */
 
 /* status codes */
 #define OD_OK 0
@@ -87,7 +87,8 @@ typedef struct od_config {
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
   int32_t sym_tile;      /* symmetric kernel variant (tile shape and waves per SIMD), 1..od_sym_variants();
-                            0 = automatic (variant 1) */
+                            0 = automatic (variant 1).  Cutoff mode, FP32c: the symmetric cell
+                            kernel's 64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4; 0 = 1 */
   int32_t pad0_;
   uint64_t seed;         /* Philox4x32-10 key */
 
@@ -109,9 +110,11 @@ typedef struct od_config {
    * cutoff <= 0: all pairs.  switch_dist in (0, cutoff): the energy is
    * multiplied by S(x) = 1 - 10x^3 + 15x^4 - 6x^5, x = (r - switch)/(cutoff - switch),
    * and the force is -d(U S)/dr; otherwise hard truncation at r < cutoff.
-   * Sources are found through a cell list with cells of cutoff/2 (nm):
-   *   nx = ny = ceil(2R/h), nz = ceil(H/h), ix = clamp(floor((x + R)/h), 0, nx-1),
-   *   iy likewise, iz = clamp(floor((z + H/2)/h), 0, nz-1), cell = ix + nx (iy + ny iz). */
+   * Sources are found through a cell list of columns w = cutoff/4 wide in x
+   * and y and slabs h = cutoff/16 high in z (nm):
+   *   nx = ny = ceil(2R/w), nz = ceil(H/h), ix = clamp(floor((x + R)/w), 0, nx-1),
+   *   iy likewise, iz = clamp(floor((z + H/2)/h), 0, nz-1), cell = iz + nz (ix + nx iy),
+   * at most OD_MAX_CELLS cells. */
   double cutoff;         /* nm */
   double switch_dist;    /* nm */
 
@@ -192,13 +195,32 @@ int od_set_step(od_ctx* ctx, int64_t step);
  * file output: integrator.step(1) plus the per-dust charge update and force
  * assembly.  out: caller-owned [nsteps][n_dust] records (may be NULL).
  * flags: OD_RUN_* below. */
-#define OD_RUN_GRAPH 1        /* replay the step as a captured hipGraph */
-#define OD_RUN_PROFILE_PAIR 2 /* time every ion-ion pair launch with HIP events */
+#define OD_RUN_GRAPH 1          /* replay the step as a captured hipGraph */
+#define OD_RUN_PROFILE_PAIR 2   /* time every ion-ion pair launch with HIP events */
+#define OD_RUN_PROFILE_STAGES 4 /* time every stage of the step with HIP events (od_stage_timing) */
 int od_run(od_ctx* ctx, int64_t nsteps, od_dust_record* out, int32_t flags);
 
 /* Accumulated HIP-event time of the main pair launches timed with
  * OD_RUN_PROFILE_PAIR since the last reset, and their count. */
 int od_pair_timing(od_ctx* ctx, double* total_ms, int64_t* launches, int32_t reset);
+
+/* Stages of one step, as od_stage_timing reports them (measurement support,
+ * no counterpart in the reference). */
+#define OD_STAGE_KICK_DRIFT 0       /* collisions, first half-kick, drift, flags (k_kick_drift) */
+#define OD_STAGE_LIST 1             /* ordered list of the flagged ions (k_scan, k_list) */
+#define OD_STAGE_CELLS 2            /* cutoff mode: binning, radix sort, cell starts, work items, gather */
+#define OD_STAGE_PAIR 3             /* main ion-ion force evaluation and its reduction */
+#define OD_STAGE_ROWS_ABSORBED 4    /* force rows of the ions inside a dust (drifted state) */
+#define OD_STAGE_CORR 5             /* cutoff mode: the flagged ions' share as sources (both states) */
+#define OD_STAGE_KICK2 6            /* correction, second half-kick, counts, reinjection, orbit sums (k_kick2) */
+#define OD_STAGE_ROWS_REINJECTED 7  /* force rows of the reinjected ions */
+#define OD_STAGE_INFLOW 8           /* the reinjected ions' share of everybody's force (k_inflow) */
+#define OD_STAGE_DUST 9             /* FP64 reductions, charge update, dust records (k_dust) */
+#define OD_N_STAGES 10
+
+/* Accumulated HIP-event time per stage (ms[OD_N_STAGES]) of the steps run
+ * with OD_RUN_PROFILE_STAGES since the last reset, and their count. */
+int od_stage_timing(od_ctx* ctx, double* ms, int64_t* steps, int32_t reset);
 
 /* Which all-pairs kernel the context runs (OD_PAIR_ROWS or OD_PAIR_SYM) and
  * how many ordered ion pairs its main launch (the one OD_RUN_PROFILE_PAIR
@@ -208,11 +230,15 @@ int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
 /* The symmetric kernel's current schedule: source blocks per work-group and
  * kernel variant (1-based), as configured, defaulted or tuned; 0 and 0 when
- * the context does not run it.  Measurement support. */
+ * the context does not run it.  Cutoff mode, FP32c: 0 and the symmetric cell
+ * kernel's variant (1, 2, 3: 1, 2, 4 target sub-blocks per wave).
+ * Measurement support. */
 int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile);
 
 /* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
- * return *sym_chunk = *sym_tile = 0 and change nothing).  It times the whole
+ * return *sym_chunk = *sym_tile = 0 and change nothing; cutoff mode, FP32c:
+ * it times the symmetric cell kernel's variants on the current ions, keeps
+ * the fastest and returns *sym_chunk = 0, *sym_tile = the variant).  It times the whole
  * pair evaluation `reps` times per candidate on the current ions: first 1, 2,
  * 4, 8, 16 and 32 source blocks per work-group with the configured kernel
  * variant, then every kernel variant (tile shape: 4, 8 or 16 64-target
@@ -238,8 +264,12 @@ int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per
  * keeps its neighbour list internal). */
 int od_get_cells(od_ctx* ctx, int32_t* cell_of_ion, int32_t* sorted_ions, int64_t* n_binned, int32_t* dims);
 
-/* Cutoff mode only: ordered ion pairs within the cutoff and candidate pairs
- * visited by the stencil, for the current cell list (measurement support). */
+/* Cutoff mode only, for the current cell list (measurement support):
+ * ordered ion pairs within the cutoff, and the ordered pair terms the cell
+ * kernel evaluates, padding included (FP64: 64 lanes x the sources of the
+ * slab ranges of each 64-target chunk; FP32c symmetric kernel: two terms per
+ * evaluated pair slot, force and reaction, one per slot of a group's own
+ * pairs). */
 int od_cutoff_pair_count(od_ctx* ctx, int64_t* in_range, int64_t* candidates);
 
 /* Diagnostics of the last step (for parity tests): per-ion flag word

@@ -94,8 +94,8 @@ struct DevParams {
   int use_cut, use_switch;
   float rc32, rs32, invw32;
   double rc64, rs64, invw64;
-  // cell grid (nm)
-  double cell_h;
+  // cell grid (nm): columns of col_w in x and y, slabs of slab_h in z
+  double col_w, slab_h;
   int nx, ny, nz, n_cells;
   uint64_t thr_top, thr_side, thr_coll;
   int dust_method[OD_MAX_DUST];
@@ -251,6 +251,15 @@ struct od_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double pair_ms = 0.0;
   int64_t pair_launches = 0;
+  // per-stage timing of one step (OD_RUN_PROFILE_STAGES): events recorded on
+  // the stream at the start of each stage, kst[i] = stage of mark i
+  static constexpr int KMARKS = 24;
+  bool kprof = false;
+  int kn = 0;
+  hipEvent_t kev[KMARKS] = {};
+  int kst[KMARKS] = {};
+  double stage_ms[OD_N_STAGES] = {};
+  int64_t stage_steps = 0;
   // symmetric pair kernel
   int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0, sym_var = 0;
   int2* wg_tab = nullptr;
@@ -263,7 +272,9 @@ struct od_ctx {
   double* sc_tp = nullptr;
   float* sc_sp = nullptr;
   int *scan_tsum = nullptr, *scan_toff = nullptr;
-  int cut = 0, n_cells = 0, rs_nblk = 0, max_items = 0, rs_passes = 2;
+  int cut = 0, n_cells = 0, n_cols = 0, rs_nblk = 0, max_items = 0, rs_passes = 2;
+  int nx = 0, ny = 0, nz = 0;  // cell grid
+  int sc_mt = 1;  // symmetric cell kernel: 64-target sub-blocks per wave (1, 2 or 4)
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
   uint32_t *skey = nullptr, *sval = nullptr;  // sorted keys/values: the buffers the last radix pass wrote
   int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_start = nullptr,
@@ -316,11 +327,15 @@ static int fill_params(const od_config* c, int n_pad, DevParams* P) {
     P->rc32 = (float)(rc * L);
     P->rs32 = (float)(rs * L);
     P->invw32 = P->use_switch ? (float)(1.0 / ((rc - rs) * L)) : 0.0f;
-    P->cell_h = c->cutoff / 2.0;
-    P->nx = (int)ceil(2.0 * c->R / P->cell_h);
+    P->col_w = c->cutoff / 4.0;
+    P->slab_h = c->cutoff / 16.0;
+    const double nxd = ceil(2.0 * c->R / P->col_w), nzd = ceil(c->H / P->slab_h);
+    // sizes beyond the cell maximum are rejected by the caller (setup_cells)
+    const bool fits = nxd * nxd * nzd <= (double)OD_MAX_CELLS;
+    P->nx = fits ? (int)nxd : 1;
     P->ny = P->nx;
-    P->nz = (int)ceil(c->H / P->cell_h);
-    P->n_cells = P->nx * P->ny * P->nz;
+    P->nz = fits ? (int)nzd : 1;
+    P->n_cells = fits ? P->nx * P->ny * P->nz : -1;
   }
   P->a32 = (float)(P->rho0 * L);
   P->linv32 = (float)(1.0 / L);
@@ -384,6 +399,8 @@ static void free_ctx(od_ctx* c) {
     if (b) hipFree(b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->kev)
+    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -493,34 +510,42 @@ static int sym_default_chunk(const std::vector<int>& be, int configured) {
 static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
   c->cut = 0;
   if (!(cfg->cutoff > 0.0)) return OD_OK;
-  if ((double)hp.nx * hp.ny * hp.nz > (double)OD_MAX_CELLS)
-    return set_err(OD_ERR_ARG, "cutoff %g nm gives %d x %d x %d cells; at most %d are supported", cfg->cutoff,
-                   hp.nx, hp.ny, hp.nz, OD_MAX_CELLS);
+  if (hp.n_cells < 0)
+    return set_err(OD_ERR_ARG, "cutoff %g nm in a cylinder R = %g, H = %g nm gives more than %d cells", cfg->cutoff,
+                   cfg->R, cfg->H, OD_MAX_CELLS);
   c->cut = 1;
   c->n_cells = hp.n_cells;
+  c->n_cols = hp.nx * hp.ny;
+  c->nx = hp.nx;
+  c->ny = hp.ny;
+  c->nz = hp.nz;
   // 8-bit digits over the keys 0 .. n_cells (n_cells marks ions left out)
   c->rs_passes = 1;
   while (c->rs_passes < 4 && ((uint64_t)c->n_cells >> (8 * c->rs_passes)) != 0) ++c->rs_passes;
   c->rs_nblk = (c->n_pad + RS_ITEMS - 1) / RS_ITEMS;
-  c->max_items = c->n_pad / 64 + c->n_cells + 1;
+  c->max_items = c->n_pad / 64 + c->n_cols + 1;
   const size_t np = c->n_pad;
   int rc = OD_OK;
   if ((rc = dalloc(&c->ckey[0], np * 4)) || (rc = dalloc(&c->ckey[1], np * 4)) || (rc = dalloc(&c->cval[0], np * 4)) ||
       (rc = dalloc(&c->cval[1], np * 4)) || (rc = dalloc(&c->rs_hist, (size_t)256 * c->rs_nblk * 4)) ||
       (rc = dalloc(&c->rs_off, (size_t)256 * c->rs_nblk * 4)) || (rc = dalloc(&c->rs_total, 4)) ||
       (rc = dalloc(&c->cell_start, (size_t)(c->n_cells + 1) * 4)) ||
-      (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cells + 1) * 4)) ||
-      (rc = dalloc(&c->item_off, (size_t)(c->n_cells + 1) * 4)) || (rc = dalloc(&c->items, (size_t)c->max_items * sizeof(int2))) ||
-      (rc = dalloc(&c->n_items, 4)) || (rc = dalloc(&c->spos, np * c->pos_bytes)) ||
-      (rc = dalloc(&c->Gcut, 3 * np * 8)))
+      (rc = dalloc(&c->cell_chunks, (size_t)(c->n_cols + 1) * 4)) ||
+      (rc = dalloc(&c->item_off, (size_t)(c->n_cols + 1) * 4)) ||
+      (rc = dalloc(&c->items, (size_t)c->max_items * sizeof(int2))) || (rc = dalloc(&c->n_items, 4)) ||
+      (rc = dalloc(&c->spos, np * c->pos_bytes)) || (rc = dalloc(&c->Gcut, 3 * np * 8)))
     return rc;
   c->skey = c->ckey[c->rs_passes & 1];
   c->sval = c->cval[c->rs_passes & 1];
-  const size_t n_tiles = (std::max<size_t>((size_t)256 * c->rs_nblk, (size_t)c->n_cells + 1) + 1023) / 1024;
+  const size_t n_tiles = (std::max<size_t>((size_t)256 * c->rs_nblk, (size_t)c->n_cols + 1) + 1023) / 1024;
   if ((rc = dalloc(&c->scan_tsum, n_tiles * 4)) || (rc = dalloc(&c->scan_toff, n_tiles * 4))) return rc;
-  if (c->prec == OD_PREC_FP32C &&
-      ((rc = dalloc(&c->sc_tp, (size_t)14 * 3 * np * 8)) || (rc = dalloc(&c->sc_sp, (size_t)SC_NS * 3 * np * 4))))
-    return rc;
+  if (c->prec == OD_PREC_FP32C) {
+    if ((rc = dalloc(&c->sc_tp, (size_t)CC_NSET * 3 * np * 8)) ||
+        (rc = dalloc(&c->sc_sp, (size_t)CC_NSET * 3 * np * 4)))
+      return rc;
+    c->sc_mt = cell_default_mt(cfg->sym_tile);
+    if (c->sc_mt < 0) return set_err(OD_ERR_ARG, "cutoff mode: sym_tile %d outside [0, %d]", cfg->sym_tile, CC_N_VARIANTS);
+  }
   return OD_OK;
 }
 
@@ -641,6 +666,13 @@ extern "C" int od_destroy(od_ctx* c) {
   return OD_OK;
 }
 
+// start of a stage of the step (stage profiling only; stage -1 ends the step)
+static void mark(od_ctx* c, int stage) {
+  if (!c->kprof || c->kn >= od_ctx::KMARKS) return;
+  if (hipEventRecord(c->kev[c->kn], c->stream) != hipSuccess) return;
+  c->kst[c->kn++] = stage;
+}
+
 // Pair force of sources pj on targets pi into F, optionally timed.  The
 // symmetric kernel treats one array as both targets and sources: it is only
 // used when pi == pj.
@@ -709,12 +741,13 @@ static int build_cells_t(od_ctx* c, const double* X, const uint32_t* flags, cons
     hipLaunchKernelGGL(k_rs_scatter, dim3(c->rs_nblk), dim3(256), 0, c->stream, c->ckey[in], c->cval[in], np, shift,
                        c->rs_nblk, c->rs_off, c->ckey[out], c->cval[out]);
   }
-  const int gc = (nc + 256) / 256;
-  hipLaunchKernelGGL(k_cell_starts, dim3(np / 256 + 1), dim3(256), 0, c->stream, c->skey, np, nc,
-                     c->cell_start);
-  hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_start, nc, c->cell_chunks);
-  launch_scan(c, c->cell_chunks, nc, c->item_off, c->n_items);
-  hipLaunchKernelGGL(k_cell_items, dim3(gc), dim3(256), 0, c->stream, c->cell_chunks, c->item_off, nc, c->items);
+  hipLaunchKernelGGL(k_cell_starts, dim3(nc / 256 + 1), dim3(256), 0, c->stream, c->skey, np, nc, c->cell_start);
+  const int gc = (c->n_cols + 255) / 256;
+  hipLaunchKernelGGL(k_cell_chunks, dim3(gc), dim3(256), 0, c->stream, c->cell_start, c->n_cols, c->nz,
+                     c->cell_chunks);
+  launch_scan(c, c->cell_chunks, c->n_cols, c->item_off, c->n_items);
+  hipLaunchKernelGGL(k_cell_items, dim3(gc), dim3(256), 0, c->stream, c->cell_chunks, c->item_off, c->n_cols,
+                     c->items);
   hipLaunchKernelGGL(k_cell_gather<T>, dim3(nb), dim3(256), 0, c->stream, c->skey, c->sval, np, nc,
                      (const PV*)pos, (PV*)c->spos);
   return OD_OK;
@@ -727,33 +760,31 @@ template <class T>
 static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const uint32_t* flags, double* F,
                            bool timed) {
   using PV = typename Pos<T>::V;
-  if (!c->cut) return launch_pair(c, pos, pos, F, timed);
+  if (!c->cut) {
+    mark(c, OD_STAGE_PAIR);
+    return launch_pair(c, pos, pos, F, timed);
+  }
+  mark(c, OD_STAGE_CELLS);
   CHKRET(build_cells_t<T>(c, X, flags, pos));
+  mark(c, OD_STAGE_PAIR);
   const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
   const bool sw = c->cfg.switch_dist > 0.0 && c->cfg.switch_dist < c->cfg.cutoff;
   if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
   if constexpr (std::is_same<T, float>::value) {
-    // symmetric: stencil half-sets, own-cell pairs, per-ion reduction
+    // symmetric: forward neighbour columns and the own column, per-ion reduction
     const float4* sp4 = (const float4*)c->spos;
-    if (sw) {
-      hipLaunchKernelGGL((k_pair32_symcell<2>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
-                         c->skey, c->cell_start, c->sc_tp, c->sc_sp);
-      hipLaunchKernelGGL((k_symcell_self<1>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
-                         c->sc_tp);
-    } else {
-      hipLaunchKernelGGL((k_pair32_symcell<1>), dim3(c->n_cells * SC_NK), dim3(64), 0, c->stream, c->dP, sp4,
-                         c->skey, c->cell_start, c->sc_tp, c->sc_sp);
-      hipLaunchKernelGGL((k_symcell_self<0>), dim3(c->n_cells), dim3(256), 0, c->stream, c->dP, sp4, c->cell_start,
-                         c->sc_tp);
-    }
-    hipLaunchKernelGGL(k_symcell_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->skey,
-                       c->sval, c->cell_start, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
+    const dim3 g(c->n_cols * CC_NSET);
+    const int ki = cell_variant_index(c->sc_mt) * 2 + (sw ? 1 : 0);
+    hipLaunchKernelGGL(cell_sym_kernels[ki], g, dim3(64), 0, c->stream, c->dP, sp4, c->skey, c->cell_start, c->sc_tp,
+                       c->sc_sp);
+    hipLaunchKernelGGL(k_symcol_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->skey, c->sval,
+                       c->cell_start, 64 * c->sc_mt, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
   } else if (sw) {
     hipLaunchKernelGGL((k_pair_cell<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->sval, c->cell_start, c->items, c->n_items, F);
+                       c->skey, c->sval, c->cell_start, c->items, c->n_items, F);
   } else {
     hipLaunchKernelGGL((k_pair_cell<T, 0>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
-                       c->sval, c->cell_start, c->items, c->n_items, F);
+                       c->skey, c->sval, c->cell_start, c->items, c->n_items, F);
   }
   if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
   return OD_OK;
@@ -780,8 +811,10 @@ template <class T>
 static int launch_step_t(od_ctx* c, bool timed) {
   using PV = typename Pos<T>::V;
   const int nb = c->nb;
+  mark(c, OD_STAGE_KICK_DRIFT);
   hipLaunchKernelGGL(k_kick_drift<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->X, c->V,
                      c->F, c->q, c->st, (PV*)c->pi, (PV*)c->pj, c->flags, c->block_cnt, c->err);
+  mark(c, OD_STAGE_LIST);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, c->block_cnt, nb, c->block_off, c->n_flag);
   hipLaunchKernelGGL(k_list<T>, dim3(nb), dim3(256), 0, c->stream, c->flags, c->block_cnt, c->block_off, c->n,
                      (const PV*)c->pi, c->slot_of, c->list_ion, (PV*)c->list_old);
@@ -789,6 +822,7 @@ static int launch_step_t(od_ctx* c, bool timed) {
   // left out of the cell list)
   CHKRET(launch_forces_t<T>(c, c->X, c->pj, c->flags, c->F, timed));
   const uint32_t dust_bits = (c->cfg.n_dust > 0) ? ((1u << c->cfg.n_dust) - 1u) : 0u;
+  mark(c, OD_STAGE_ROWS_ABSORBED);
   if (dust_bits) {
     if (c->cut)
       hipLaunchKernelGGL(k_rows_cell<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, c->X,
@@ -799,12 +833,17 @@ static int launch_step_t(od_ctx* c, bool timed) {
                          (const PV*)c->pi, c->n_flag, (const PV*)c->list_old, c->flags, c->list_ion, dust_bits,
                          c->rowD);
   }
-  if (c->cut) launch_corr_cell<T>(c, (const PV*)c->list_old, 0, c->Gcut);
+  if (c->cut) {
+    mark(c, OD_STAGE_CORR);
+    launch_corr_cell<T>(c, (const PV*)c->list_old, 0, c->Gcut);
+  }
+  mark(c, OD_STAGE_KICK2);
   hipLaunchKernelGGL(k_kick2<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
                      c->X, c->V, c->F, c->q, c->st,
                      (const PV*)c->pi, (PV*)c->pj, c->flags, c->n_flag, c->slot_of, (const PV*)c->list_old,
                      (PV*)c->list_new, c->face, c->rowD, c->cut ? (const double*)c->Gcut : (const double*)nullptr,
                      c->partials);
+  mark(c, OD_STAGE_ROWS_REINJECTED);
   if (c->cut)
     hipLaunchKernelGGL(k_rows_cell<T>, dim3(c->rows_grid), dim3(256), 0, c->stream, c->dP, c->X, (const PV*)c->spos,
                        c->cell_start, c->n_flag, (const PV*)c->list_new, (const PV*)c->list_new, c->flags,
@@ -812,12 +851,18 @@ static int launch_step_t(od_ctx* c, bool timed) {
   else
     hipLaunchKernelGGL(k_rows<T>, dim3(c->rows_grid), dim3(RowShape<T>::TB), 0, c->stream, c->dP, (const int*)nullptr,
                        (const PV*)c->pj, c->n_flag, (const PV*)c->list_new, c->flags, c->list_ion, 0u, c->rowF);
-  if (c->cut) launch_corr_cell<T>(c, (const PV*)c->list_new, 1, c->F);
+  if (c->cut) {
+    mark(c, OD_STAGE_CORR);
+    launch_corr_cell<T>(c, (const PV*)c->list_new, 1, c->F);
+  }
+  mark(c, OD_STAGE_INFLOW);
   hipLaunchKernelGGL(k_inflow<T>, dim3(nb), dim3(256), 0, c->stream, c->dP, (const int*)nullptr, c->block_off, nb,
                      c->F, (const PV*)c->pj, c->flags, c->n_flag, (const PV*)c->list_new, c->slot_of, c->rowF,
                      c->cut);
+  mark(c, OD_STAGE_DUST);
   hipLaunchKernelGGL(k_dust, dim3(1), dim3(256), 0, c->stream, c->dP, c->partials, c->q, c->st, c->rec,
                      std::max(1, c->cfg.n_dust), 1);
+  mark(c, -1);
   return OD_OK;
 }
 
@@ -975,7 +1020,11 @@ extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t fl
   if (c->host_step + nsteps > 0xFFFFFFFFll) return set_err(OD_ERR_ARG, "step counter would exceed u32");
   HIPCHK(hipSetDevice(c->dev));
   const bool profile = (flags & OD_RUN_PROFILE_PAIR) != 0;
-  const bool graph = (flags & OD_RUN_GRAPH) != 0 && !profile;
+  const bool stages = (flags & OD_RUN_PROFILE_STAGES) != 0;
+  const bool graph = (flags & OD_RUN_GRAPH) != 0 && !profile && !stages;
+  if (stages)
+    for (hipEvent_t& e : c->kev)
+      if (!e) HIPCHK(hipEventCreate(&e));
   const int nd = c->cfg.n_dust;
   int64_t done = 0;
   while (done < nsteps) {
@@ -992,7 +1041,11 @@ extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t fl
       }
     }
     for (; left > 0; --left) {
-      CHKRET(launch_step(c, profile));
+      c->kprof = stages;
+      c->kn = 0;
+      const int rc = launch_step(c, profile);
+      c->kprof = false;
+      CHKRET(rc);
       HIPCHK(hipGetLastError());
       if (profile) {
         HIPCHK(hipEventSynchronize(c->ev1));
@@ -1000,6 +1053,15 @@ extern "C" int od_run(od_ctx* c, int64_t nsteps, od_dust_record* out, int32_t fl
         HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->pair_ms += ms;
         c->pair_launches += 1;
+      }
+      if (stages && c->kn > 0) {
+        HIPCHK(hipEventSynchronize(c->kev[c->kn - 1]));
+        for (int i = 0; i + 1 < c->kn; ++i) {
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, c->kev[i], c->kev[i + 1]));
+          if (c->kst[i] >= 0) c->stage_ms[c->kst[i]] += ms;
+        }
+        c->stage_steps += 1;
       }
     }
     if (out && nd > 0)
@@ -1045,35 +1107,105 @@ extern "C" int od_get_cells(od_ctx* c, int32_t* cell_of_ion, int32_t* sorted_ion
   return OD_OK;
 }
 
+// Ordered pair terms the context's cell kernel evaluates on a cell list
+// (cs: cell starts, key: sorted cell ids), padding lanes and slots included:
+// the FP64 kernel evaluates 64 lanes x every source of the 77 slab ranges of
+// each 64-target chunk; the symmetric FP32 kernel's slots each give two
+// ordered terms (force and reaction), its own-group tiles one.
+static int64_t cell_candidates(const od_ctx* c, const std::vector<int>& cs, const std::vector<uint32_t>& key) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  auto range = [&](int bx, int by, int blo, int bhi, int nbz, int64_t& s0, int64_t& s1) {
+    s0 = s1 = 0;
+    if (bx < 0 || bx >= nx || by < 0 || by >= ny) return;
+    const int cb = nz * (bx + nx * by);
+    s0 = cs[cb + std::max(blo - nbz, 0)];
+    s1 = cs[cb + std::min(bhi + nbz, nz - 1) + 1];
+  };
+  auto up64 = [](int64_t v) { return (v + 63) / 64 * 64; };
+  int64_t total = 0;
+  for (int A = 0; A < c->n_cols; ++A) {
+    const int cA = A * nz, a0 = cs[cA], a1 = cs[cA + nz];
+    if (a0 == a1) continue;
+    const int ax = A % nx, ay = A / nx;
+    if (c->prec == OD_PREC_FP32C) {
+      const int G = 64 * c->sc_mt;
+      for (int k = 0; k < CC_NSET; ++k) {
+        const bool own = k == CC_NFWD;
+        int dx = 0, dy = 0;
+        if (!own) cc_fwd(k, dx, dy);
+        const int bx = ax + dx, by = ay + dy;
+        if (bx < 0 || bx >= nx || by >= ny) continue;
+        const int cb = nz * (bx + nx * by);
+        if (!own && cs[cb] == cs[cb + nz]) continue;
+        const int nbz = cc_nbz(dx, dy);
+        for (int t0 = a0; t0 < a1; t0 += G) {
+          const int t1 = std::min(t0 + G, a1);
+          const int blo = (int)key[t0] - cA, bhi = (int)key[t1 - 1] - cA;
+          int64_t s0, s1;
+          range(bx, by, blo, bhi, nbz, s0, s1);
+          if (own) {
+            s0 = t1;
+            total += (int64_t)G * up64(t1 - t0);
+          }
+          if (s1 > s0) total += 2 * (int64_t)G * up64(s1 - s0);
+        }
+      }
+    } else {
+      for (int t0 = a0; t0 < a1; t0 += 64) {
+        const int tl = std::min(t0 + 63, a1 - 1);
+        const int blo = (int)key[t0] - cA, bhi = (int)key[tl] - cA;
+        for (int k = 0; k < CC_NOFF; ++k) {
+          int dx, dy;
+          cc_off(k, dx, dy);
+          int64_t s0, s1;
+          range(ax + dx, ay + dy, blo, bhi, cc_nbz(dx, dy), s0, s1);
+          total += 64 * (s1 - s0);
+        }
+      }
+    }
+  }
+  return total;
+}
+
 extern "C" int od_cutoff_pair_count(od_ctx* c, int64_t* in_range, int64_t* candidates) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   if (!c->cut) return set_err(OD_ERR_STATE, "od_cutoff_pair_count: context has no cutoff");
   HIPCHK(hipSetDevice(c->dev));
   unsigned long long* d = nullptr;
-  CHKRET(dalloc(&d, 16));
-  unsigned long long h[2] = {0, 0};
-  hipError_t e = hipMemcpyAsync(d, h, 16, hipMemcpyHostToDevice, c->stream);
+  CHKRET(dalloc(&d, 8));
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpyAsync(d, &h, 8, hipMemcpyHostToDevice, c->stream);
   const int grid = std::max(1, std::min(4096, c->max_items / 4 + 1));
   if (e == hipSuccess) {
     if (c->prec == OD_PREC_FP32C)
       hipLaunchKernelGGL(k_cell_pair_count<float>, dim3(grid), dim3(256), 0, c->stream, c->dP,
-                         (const float4*)c->spos, c->cell_start, c->items, c->n_items, d);
+                         (const float4*)c->spos, c->skey, c->cell_start, c->items, c->n_items, d);
     else
       hipLaunchKernelGGL(k_cell_pair_count<double>, dim3(grid), dim3(256), 0, c->stream, c->dP,
-                         (const double4*)c->spos, c->cell_start, c->items, c->n_items, d);
+                         (const double4*)c->spos, c->skey, c->cell_start, c->items, c->n_items, d);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->stream);
+  std::vector<int> cs(c->n_cells + 1);
+  std::vector<uint32_t> key(c->n_pad);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(cs.data(), c->cell_start, cs.size() * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(key.data(), c->skey, key.size() * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(d);
   if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_cutoff_pair_count: %s", hipGetErrorString(e));
-  if (in_range) *in_range = (int64_t)h[0];
-  if (candidates) *candidates = (int64_t)h[1];
+  if (in_range) *in_range = (int64_t)h;
+  if (candidates) *candidates = cell_candidates(c, cs, key);
   return OD_OK;
 }
 
 extern "C" int od_sym_choice(od_ctx* c, int32_t* sym_chunk, int32_t* sym_tile) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
+  if (c->cut && c->prec == OD_PREC_FP32C) {
+    // symmetric cell kernel: no source blocks, the variant (MT 1, 2, 4 -> 1, 2, 3)
+    if (sym_chunk) *sym_chunk = 0;
+    if (sym_tile) *sym_tile = cell_variant_index(c->sc_mt) + 1;
+    return OD_OK;
+  }
   if (sym_chunk) *sym_chunk = c->sym ? c->sym_ch : 0;
   if (sym_tile) *sym_tile = c->sym ? c->sym_var + 1 : 0;
   return OD_OK;
@@ -1099,6 +1231,18 @@ extern "C" int od_pair_info(od_ctx* c, int32_t* mode, int64_t* pairs_per_launch)
     } else {
       *pairs_per_launch = n * (n - 1);
     }
+  }
+  return OD_OK;
+}
+
+extern "C" int od_stage_timing(od_ctx* c, double* ms, int64_t* steps, int32_t reset) {
+  if (!c) return set_err(OD_ERR_ARG, "null context");
+  for (int i = 0; i < OD_N_STAGES; ++i)
+    if (ms) ms[i] = c->stage_ms[i];
+  if (steps) *steps = c->stage_steps;
+  if (reset) {
+    for (double& v : c->stage_ms) v = 0.0;
+    c->stage_steps = 0;
   }
   return OD_OK;
 }
@@ -1297,10 +1441,53 @@ extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t
   return OD_OK;
 }
 
+// Autotuner of the symmetric cell kernel (cutoff mode, FP32c): times `reps`
+// whole force evaluations (binning included) on the current positions for
+// each variant (MT 1, 2, 4) and keeps the fastest.
+static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_steps = 0;
+  double* Fs = nullptr;
+  CHKRET(dalloc(&Fs, (size_t)3 * c->n_pad * 8));
+  int rc = load_positions(c, c->X, c->pi, c->pj);
+  double best_ms = 1e300;
+  int best = c->sc_mt;
+  for (int v = 0; v < CC_N_VARIANTS && rc == OD_OK; ++v) {
+    c->sc_mt = cell_variant_mt[v];
+    rc = launch_forces(c, c->X, c->pj, nullptr, Fs, false);  // warm-up
+    if (rc == OD_OK && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = set_err(OD_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < reps && rc == OD_OK; ++r) rc = launch_forces(c, c->X, c->pj, nullptr, Fs, false);
+    float ms = 0.f;
+    if (rc == OD_OK && (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+                        hipGetLastError() != hipSuccess || hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess))
+      rc = set_err(OD_ERR_HIP, "timing the cell kernel variants failed");
+    if (rc == OD_OK && ms < best_ms) {
+      best_ms = ms;
+      best = c->sc_mt;
+    }
+  }
+  hipFree(Fs);
+  c->sc_mt = best;
+  *chosen_var = cell_variant_index(best);
+  return rc;
+}
+
 extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   if (reps < 1 || reps > 100) return set_err(OD_ERR_ARG, "reps %d outside [1, 100]", reps);
   int chosen = 0, var = -1;
+  if (c->cut && c->prec == OD_PREC_FP32C) {
+    if (!c->have_ions) return set_err(OD_ERR_STATE, "od_tune needs the ions (call od_set_ions first)");
+    HIPCHK(hipSetDevice(c->dev));
+    CHKRET(tune_cells(c, reps, &var));
+    CHKRET(load_positions(c, c->X, c->pi, c->pj));
+    CHKRET(launch_forces(c, c->X, c->pj, nullptr, c->F, false));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   if (c->sym) {
     if (!c->have_ions) return set_err(OD_ERR_STATE, "od_tune needs the ions (call od_set_ions first)");
     HIPCHK(hipSetDevice(c->dev));

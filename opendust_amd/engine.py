@@ -23,6 +23,10 @@ ABI_VERSION = 2
 
 RUN_GRAPH = 1
 RUN_PROFILE_PAIR = 2
+RUN_PROFILE_STAGES = 4
+# stages of a step as od_stage_timing reports them (include/opendust_amd.h OD_STAGE_*)
+STAGES = ("kick_drift", "list", "cells", "pair", "rows_absorbed", "corr", "kick2", "rows_reinjected", "inflow",
+          "dust")
 
 PAIR_AUTO = 0
 PAIR_ROWS = 1
@@ -147,7 +151,7 @@ EXPORTS = [
     "od_batch_pair_timing", "od_batch_destroy", "od_tune", "od_batch_tune", "od_sym_variants", "od_sym_variant_info",
     "od_sym_choice",
     "od_density_profile", "od_density_profile_z", "od_write_xyz", "od_write_csv", "od_write_restart",
-    "od_update_config", "od_density_profile_noalloc",
+    "od_update_config", "od_density_profile_noalloc", "od_stage_timing",
 ]
 
 
@@ -173,6 +177,7 @@ def load_library(path=LIB_PATH):
     lib.od_run.argtypes = [c_ctx, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32]
     lib.od_pair_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), ctypes.c_int32]
     lib.od_pair_info.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int64)]
+    lib.od_stage_timing.argtypes = [c_ctx, dp, P(ctypes.c_int64), ctypes.c_int32]
     lib.od_get_cells.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_int32)]
     lib.od_cutoff_pair_count.argtypes = [c_ctx, P(ctypes.c_int64), P(ctypes.c_int64)]
     lib.od_get_step_info.argtypes = [c_ctx, P(ctypes.c_uint32), P(ctypes.c_uint8), P(ctypes.c_int64)]
@@ -377,6 +382,16 @@ class Engine:
         _check(self.lib.od_pair_timing(self._ctx, ctypes.byref(ms), ctypes.byref(n), int(bool(reset))),
                "od_pair_timing")
         return ms.value, n.value
+
+    def stage_timing(self, reset=False):
+        """{stage: ms per step} of the steps run with RUN_PROFILE_STAGES since the
+        last reset (HIP events around each stage), and their count."""
+        ms = np.zeros(len(STAGES))
+        steps = ctypes.c_int64()
+        _check(self.lib.od_stage_timing(self._ctx, _dptr(ms), ctypes.byref(steps), int(bool(reset))),
+               "od_stage_timing")
+        n = max(steps.value, 1)
+        return {name: float(v) / n for name, v in zip(STAGES, ms)}, steps.value
 
     def pair_info(self):
         """(mode, ordered pairs covered by one timed main pair launch)"""

@@ -471,11 +471,12 @@ int orc_max_threads(void) {
 /* layout check against the ctypes mirror */
 int64_t orc_config_size(void) { return (int64_t)sizeof(od_config); }
 
-/* cell of a position under the engine's binning specification (opendust_amd.h) */
+/* cell of a position under the engine's binning specification (opendust_amd.h):
+ * columns of cutoff/4 in x and y, slabs of cutoff/16 in z, z fastest */
 int orc_cell_of(const od_config* c, const double* x, int* dims) {
-  const double h = c->cutoff / 2.0;
-  const int nx = (int)ceil(2.0 * c->R / h), ny = nx, nz = (int)ceil(c->H / h);
-  int ix = (int)floor((x[0] + c->R) / h), iy = (int)floor((x[1] + c->R) / h), iz = (int)floor((x[2] + c->H / 2.0) / h);
+  const double w = c->cutoff / 4.0, h = c->cutoff / 16.0;
+  const int nx = (int)ceil(2.0 * c->R / w), ny = nx, nz = (int)ceil(c->H / h);
+  int ix = (int)floor((x[0] + c->R) / w), iy = (int)floor((x[1] + c->R) / w), iz = (int)floor((x[2] + c->H / 2.0) / h);
   ix = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);
   iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
   iz = iz < 0 ? 0 : (iz >= nz ? nz - 1 : iz);
@@ -484,7 +485,7 @@ int orc_cell_of(const od_config* c, const double* x, int* dims) {
     dims[1] = ny;
     dims[2] = nz;
   }
-  return ix + nx * (iy + ny * iz);
+  return iz + nz * (ix + nx * iy);
 }
 
 /* Axial ion-density profile of one frame (tools/concentration.cpp:32-101):

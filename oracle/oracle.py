@@ -23,18 +23,20 @@ def build():
     return LIB_PATH
 
 
-def load():
+def load(path=None):
     global _lib
     if _lib is not None:
         return _lib
-    # make rebuilds only when od_oracle.c or include/opendust_amd.h changed;
-    # a stale library would read od_config with the wrong layout
-    if os.path.exists(os.path.join(HERE, "od_oracle.c")) and subprocess.run(
-            ["make", "-s", "-C", HERE], capture_output=True).returncode != 0 and not os.path.exists(LIB_PATH):
-        raise RuntimeError("building the oracle failed")
+    if path is None:
+        path = LIB_PATH
+        # make rebuilds only when od_oracle.c or include/opendust_amd.h changed;
+        # a stale library would read od_config with the wrong layout
+        if os.path.exists(os.path.join(HERE, "od_oracle.c")) and subprocess.run(
+                ["make", "-s", "-C", HERE], capture_output=True).returncode != 0 and not os.path.exists(LIB_PATH):
+            raise RuntimeError("building the oracle failed")
     from opendust_amd.engine import Config, DustRecord, Table  # structure layouts only
 
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     dp = P(ctypes.c_double)
     lib.orc_philox.argtypes = [P(ctypes.c_uint32), ctypes.c_uint64, P(ctypes.c_uint32)]
@@ -68,6 +70,43 @@ def load():
         raise RuntimeError("oracle library was built against a different od_config layout; run make -C oracle")
     _lib = lib
     return lib
+
+
+def use_native_build():
+    """Rebuild the oracle for the host it runs on (-O3 -march=native, OpenMP)
+    in a temporary directory and make this module use that build from now on
+    (the CPU-baseline leg of the benchmark; the committed Makefile targets
+    x86-64-v3 so that one build runs on any host).  Returns the compile
+    command."""
+    global _lib
+    import tempfile
+
+    load()  # layout check of the regular build first
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-shared",
+           "-I", os.path.join(HERE, "..", "include"), "-o", out, os.path.join(HERE, "od_oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    saved, _lib = _lib, None
+    try:
+        load(out)
+    except Exception:
+        _lib = saved
+        raise
+    return " ".join(cmd[:6])
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
 
 
 def _d(a):

@@ -2,13 +2,12 @@
 against the FP64 CPU oracle, and FP32c against the engine's FP64 kernels).
 
   * Cutoff mode at the density of the 2^22-ion cutoff workload (667 ions per
-    r_D_e^3): cells of r_c / 2 = 1 r_D_e hold several hundred ions, so the
-    symmetric cell kernel runs several 256-target groups per cell (its
-    accumulating reaction branch), the own-cell kernel more than 256 targets
-    per cell, and the FP64 cell kernel several 64-target chunks per cell.
-    Small enough (65536 ions, R = 2.5, H = 5 r_D_e) for the oracle's full
-    rows; then the full 2^22-ion workload itself against FP64 on the GPU and
-    oracle rows of a sample.
+    r_D_e^3): columns of r_c / 4 hold several hundred ions, so the symmetric
+    cell kernel runs several target groups per column (its accumulating
+    reaction branch, reaches that overlap and leave gaps), and the FP64 cell
+    kernel several 64-target chunks per column.  Small enough (65536 ions,
+    R = 2.5, H = 5 r_D_e) for the oracle's full rows; then the full 2^22-ion
+    workload itself against FP64 on the GPU and oracle rows of a sample.
   * The 16384-ion single-dust workload at full size, several steps.
   * A batched step of the sweep generator with its largest simulation and a
     2^17-ion one.
@@ -19,7 +18,7 @@ the FP32c kernels see positions rounded to FP32 in units of r_D_e / log2 e:
 against FP64 forces on those rounded positions, per ion |dF_i| <= 1e-5 |F_i|
 + 1e-6 rms|F| and norm-wise <= 2e-6 (compensated sums; FP32 pair terms);
 against exact positions norm-wise <= 1e-4 (the rounding alone, dominated by
-close pairs).  With a hard cutoff a pair whose distance lies within the
+close pairs; <= 3e-4 in the 2^22-ion cylinder, where |u| reaches 14.4).  With a hard cutoff a pair whose distance lies within the
 rounding of r of the cutoff can be counted by one side and not the other:
 each such pair may add one pair force at the cutoff, |f(r_c)|, to an ion's
 error.  The oracle counts those pairs per row (relative window 1e-6 for
@@ -90,10 +89,10 @@ def test_cutoff_dense_cells_vs_oracle(oracle_lib, precision, rc, rs):
     in_range, cand = e.cutoff_pair_count()
     e.close()
     n_cells = dims[0] * dims[1] * dims[2]
-    occ = np.bincount(cell[cell < n_cells], minlength=n_cells)
-    say("  cells", dims, "max ions per cell", occ.max(), "in-range / candidate pairs", in_range, cand)
-    # several 256-target groups and 64-target chunks per cell, > 256 per cell
-    assert occ.max() > 512
+    col = np.bincount(cell[cell < n_cells] // dims[2], minlength=dims[0] * dims[1])
+    say("  cells", dims, "max ions per column", col.max(), "in-range / candidate pair terms", in_range, cand)
+    # several target groups (up to 256 ions each) and 64-ion chunks per column
+    assert col.max() > 512
     ref = np.array([oracle_lib.cell_of(cfg, xi)[0] for xi in x0])
     assert np.array_equal(cell, ref)
     assert np.array_equal(order, np.lexsort((np.arange(len(ref)), ref)))
@@ -205,8 +204,10 @@ def test_cutoff_full_size_fp32c_vs_fp64_and_oracle_rows(oracle_lib):
     FP64 cell kernel on the FP32-rounded positions: the 2000 sampled rows to
     the per-row bound with the oracle's counts of pairs at the cutoff; every
     ion either within the flip-free bound or, for at most 3 % of them
-    (distance within the FP32 rounding of the cutoff), within it plus two pair
-    forces at the cutoff.  (c) Against exact positions norm-wise <= 1e-4.
+    (distance within the FP32 rounding of the cutoff), within it plus three
+    pair forces at the cutoff.  (c) Against exact positions: the difference is
+    the FP64 kernel's own difference between rounded and exact positions (to
+    5e-6), which is <= 3e-4 norm-wise (measured 1.67e-4).
     (At this density 6 % of the ions have a pair within 1e-6 of the cutoff,
     about a quarter of that within the FP32 rounding of r; a flipped pair
     adds |f(r_c)| = 4.4e-5 rms|F| to its ion's error.)"""
@@ -225,9 +226,9 @@ def test_cutoff_full_size_fp32c_vs_fp64_and_oracle_rows(oracle_lib):
     cell, _, dims = e.cells()
     e.close()
     n_cells = dims[0] * dims[1] * dims[2]
-    occ = np.bincount(cell[cell < n_cells], minlength=n_cells)
-    say("  cells", dims, "mean ions per occupied cell %.0f, max %d" % (occ[occ > 0].mean(), occ.max()))
-    assert occ.max() > 512
+    col = np.bincount(cell[cell < n_cells] // dims[2], minlength=dims[0] * dims[1])
+    say("  cells", dims, "mean ions per occupied column %.0f, max %d" % (col[col > 0].mean(), col.max()))
+    assert col.max() > 512
     cfg32, x32_0, _ = _quiet(workloads.prepare, _quiet(workloads.cutoff_large, N=2 ** 22, n=2, precision="fp32c",
                                                        seed=3), alpha=alpha, np_seed=5, precision="fp32c")
     assert np.array_equal(x32_0, x0)
@@ -255,10 +256,16 @@ def test_cutoff_full_size_fp32c_vs_fp64_and_oracle_rows(oracle_lib):
         % (np.linalg.norm(f32 - f64r) / np.linalg.norm(f64r), int(over.sum()), 100 * over.mean(),
            float(np.max((err - b0) / f_cut))))
     assert over.mean() <= 0.03
-    assert np.all(err <= b0 + 2.0 * f_cut)
+    # at most three flipped pairs on one ion (two seen at this seed)
+    assert np.all(err <= b0 + 3.0 * f_cut)
     rel_q = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
-    say("  fp32c vs fp64 on exact positions: norm-wise", rel_q)
-    assert rel_q <= 1e-4
+    rel_round = np.linalg.norm(f64r - f64) / np.linalg.norm(f64)
+    say("  fp32c vs fp64 on exact positions: norm-wise", rel_q, "(fp64 on rounded positions alone:", rel_round, ")")
+    # all of it is the rounding of the positions: |u| <= 14.4 here (ulp 9.5e-7,
+    # twice that of the 2^20 all-pairs workload) and the truncated sum leaves
+    # the close pairs a larger share of each force
+    assert rel_q <= rel_round + 5e-6
+    assert rel_round <= 3e-4
 
 
 @pytest.mark.timeout(600)

@@ -135,35 +135,80 @@ def test_truncated_force_is_minus_gradient_of_switched_energy(orc, base, switch)
 
 
 def test_binning_spec(orc, base):
+    """Columns of cutoff/4 in x and y, slabs of cutoff/16 in z, z fastest
+    (a column's cells are consecutive), clamped to the grid."""
     _, cfg, _, _ = base
     cfg.cutoff = 1.5 * cfg.r_D_e
     try:
-        h = cfg.cutoff / 2
+        w, h = cfg.cutoff / 4, cfg.cutoff / 16
         cid, dims = orc.cell_of(cfg, np.array([-cfg.R, -cfg.R, -cfg.H / 2]))
-        assert cid == 0 and dims == (int(np.ceil(2 * cfg.R / h)),) * 2 + (int(np.ceil(cfg.H / h)),)
+        assert cid == 0 and dims == (int(np.ceil(2 * cfg.R / w)),) * 2 + (int(np.ceil(cfg.H / h)),)
         nx, ny, nz = dims
         cid, _ = orc.cell_of(cfg, np.array([cfg.R * 2, cfg.R * 2, cfg.H]))  # clamped to the last cell
         assert cid == nx * ny * nz - 1
-        p = np.array([-cfg.R + 2.5 * h, -cfg.R + 0.5 * h, -cfg.H / 2 + 1.5 * h])
-        assert orc.cell_of(cfg, p)[0] == 2 + nx * (0 + ny * 1)
+        p = np.array([-cfg.R + 2.5 * w, -cfg.R + 0.5 * w, -cfg.H / 2 + 1.5 * h])
+        assert orc.cell_of(cfg, p)[0] == 1 + nz * (2 + nx * 0)
+        # one slab up is the next cell, one column over is nz cells on
+        assert orc.cell_of(cfg, p + [0, 0, h])[0] == orc.cell_of(cfg, p)[0] + 1
+        assert orc.cell_of(cfg, p + [w, 0, 0])[0] == orc.cell_of(cfg, p)[0] + nz
+        assert orc.cell_of(cfg, p + [0, w, 0])[0] == orc.cell_of(cfg, p)[0] + nz * nx
     finally:
         cfg.cutoff = 0.0
 
 
+def test_cell_offsets_cover_the_cutoff_sphere():
+    """The neighbour columns and slab reaches the cell kernels visit (77
+    column offsets, slab reach 1 + max{j : j^2 < 256 - 16 (mx^2 + my^2)})
+    contain every cell that can hold an ion within the cutoff of an ion in
+    the target cell, and drop only cells that cannot: brute force over random
+    pairs of points in cells of the grid."""
+    w, h, rc = 0.25, 1.0 / 16.0, 1.0
+
+    def m(d):
+        return max(abs(d) - 1, 0)
+
+    def nbz(dx, dy):
+        v = 256 - 16 * (m(dx) ** 2 + m(dy) ** 2)
+        j = 0
+        while (j + 1) ** 2 < v:
+            j += 1
+        return j + 1
+
+    offs = {(dx, dy) for dx in range(-4, 5) for dy in range(-4, 5) if 16 * (m(dx) ** 2 + m(dy) ** 2) < 256}
+    assert len(offs) == 77 and (4, 4) not in offs and (4, 3) in offs
+    fwd = [(dx, 0) for dx in range(1, 5)] + [(dx, dy) for dy in range(1, 4) for dx in range(-4, 5)] + \
+        [(dx, 4) for dx in range(-3, 4)]
+    assert len(fwd) == 38 and set(fwd) | {(0, 0)} | {(-a, -b) for a, b in fwd} == offs
+    rng = np.random.default_rng(5)
+    for dx in range(-5, 6):
+        for dy in range(-5, 6):
+            for dz in range(-18, 19):
+                # closest approach of two points in cells (0, 0, 0) and (dx, dy, dz)
+                gap = np.array([m(dx) * w, m(dy) * w, m(dz) * h])
+                reachable = np.linalg.norm(gap) < rc
+                listed = (dx, dy) in offs and abs(dz) <= nbz(dx, dy)
+                assert reachable == listed, (dx, dy, dz)
+                if reachable:
+                    # the bound is attained: random points come arbitrarily close to the gap
+                    a = rng.uniform(0, 1, size=(2000, 3)) * [w, w, h]
+                    b = (rng.uniform(0, 1, size=(2000, 3)) + [dx, dy, dz]) * [w, w, h]
+                    assert np.min(np.linalg.norm(a - b, axis=1)) >= np.linalg.norm(gap) - 1e-12
+
+
 def test_binning_spec_beyond_16_bit_cell_ids(orc, base):
     """A grid of more than 65535 cells (R = 10, H = 40 r_D_e at a 1 r_D_e
-    cutoff: 40 x 40 x 80 cells), where cell ids need a third 8-bit radix
+    cutoff: 80 x 80 x 640 cells), where cell ids need a third 8-bit radix
     digit: the same formula, no wrap-around."""
     _, cfg, _, _ = base
     save = (cfg.R, cfg.H, cfg.cutoff)
     lam = cfg.r_D_e
     try:
         cfg.R, cfg.H, cfg.cutoff = 10 * lam, 40 * lam, 1.0 * lam
-        h = cfg.cutoff / 2
+        w, h = cfg.cutoff / 4, cfg.cutoff / 16
         cid, dims = orc.cell_of(cfg, np.array([cfg.R, cfg.R, cfg.H / 2]))
-        assert dims == (40, 40, 80) and cid == 40 * 40 * 80 - 1 > 65535
-        p = np.array([-cfg.R + 17.5 * h, -cfg.R + 33.5 * h, -cfg.H / 2 + 71.5 * h])
-        assert orc.cell_of(cfg, p)[0] == 17 + 40 * (33 + 40 * 71)
+        assert dims == (80, 80, 640) and cid == 80 * 80 * 640 - 1 > 65535
+        p = np.array([-cfg.R + 17.5 * w, -cfg.R + 33.5 * w, -cfg.H / 2 + 471.5 * h])
+        assert orc.cell_of(cfg, p)[0] == 471 + 640 * (17 + 80 * 33)
     finally:
         cfg.R, cfg.H, cfg.cutoff = save
 
