@@ -52,6 +52,12 @@ STAGE_BYTES_PER_ION = {
 }
 
 
+# cutoff mode: k_kick2 also reads the flagged sources' share G (24); k_inflow
+# only reads the flags and writes the reinjected ions' rows (the kept ions'
+# share of the reinjected sources is added by the cell correction kernel)
+STAGE_BYTES_PER_ION_CUTOFF = dict(STAGE_BYTES_PER_ION, kick2=141, inflow=4)
+
+
 def cell_stage_bytes_per_ion(n_cells):
     """Binning (read X 24 + flags 4, write key and value 8), one 8-bit radix
     pass per digit of the cell ids (read key 4 for the histogram, read and
@@ -186,7 +192,7 @@ def cutoff_leg(steps):
         "cell_kernel_variant": {"sym_tile": variant, "targets_per_lane": {1: 1, 2: 2, 3: 4}.get(variant)},
         "grid": {"dims": list(dims), "cells": n_cells, "radix_passes": passes},
         "stage_ms": st,
-        "hbm_stages": hbm_stages(st, 2 ** 22, dict(STAGE_BYTES_PER_ION, cells=cb)),
+        "hbm_stages": hbm_stages(st, 2 ** 22, dict(STAGE_BYTES_PER_ION_CUTOFF, cells=cb)),
     }
 
 
@@ -252,6 +258,28 @@ def sweep_leg(steps, seq_steps, n_sims=64, sweep_len=5000):
         "tuned_sym_chunk": chunk,
         "tuned_sym_tile": tile,
     }
+
+
+def issue_probe():
+    """Build and run opendust_amd/csrc/od_issue_probe.hip: the VALU issue
+    rate of full-rate FMAs, of the transcendental ops alone and after FMAs,
+    and of the pair term with and without its three transcendental ops (the
+    measured ceiling of the pair kernels)."""
+    import subprocess
+    import tempfile
+
+    from opendust_amd import build
+
+    exe = os.path.join(tempfile.mkdtemp(prefix="od_probe_"), "od_issue_probe")
+    cmd = [build.hipcc(), "--offload-arch=" + build.ARCH, "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-o", exe, os.path.join(ROOT, "opendust_amd", "csrc", "od_issue_probe.hip")]
+    subprocess.run(cmd, check=True, capture_output=True)
+    res = json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
+    fr, tr = res["law_fullrate"], res["law_trans"]
+    # 67 VALU instructions per loop iteration of 4 pair terms in both laws
+    res["law_trans_slots_per_transcendental"] = 1.0 + (tr["ms"] / fr["ms"] - 1.0) * 67.0 / 12.0
+    print(json.dumps(res))
+    return res
 
 
 def sweep_full(n_sims=64, sweep_len=5000, seq_steps=200, out_path=None):
@@ -332,6 +360,8 @@ def main():
                     help="symmetric kernel variant (1-based; tile shape and waves per SIMD); 0: autotune, or "
                          "the size-dependent default with --no-tune or --sym-chunk")
     ap.add_argument("--no-tune", action="store_true", help="skip the symmetric kernel's autotuner")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only: the VALU issue-rate probe (opendust_amd/csrc/od_issue_probe.hip)")
     ap.add_argument("--sweep-full", action="store_true",
                     help="only: the whole 64-simulation, 5000-step sweep end to end, batched and sequential")
     args = ap.parse_args()
@@ -345,6 +375,9 @@ def main():
         return
     if args.sweep_full:
         sweep_full()
+        return
+    if args.issue_probe:
+        issue_probe()
         return
 
     from opendust_amd import engine, workloads
