@@ -271,8 +271,10 @@ def issue_probe():
     from opendust_amd import build
 
     exe = os.path.join(tempfile.mkdtemp(prefix="od_probe_"), "od_issue_probe")
-    cmd = [build.hipcc(), "--offload-arch=" + build.ARCH, "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
-           "-Xclang", "-packed-fp32-ops", "-o", exe, os.path.join(ROOT, "opendust_amd", "csrc", "od_issue_probe.hip")]
+    csrc = os.path.join(ROOT, "opendust_amd", "csrc")
+    cmd = [build.hipcc(), "--offload-arch=" + build.ARCH, "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang",
+           "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-I", csrc, "-o", exe,
+           os.path.join(csrc, "od_issue_probe.hip")]
     subprocess.run(cmd, check=True, capture_output=True)
     res = json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
     fr, tr = res["law_fullrate"], res["law_trans"]
@@ -364,6 +366,8 @@ def main():
                     help="only: the VALU issue-rate probe (opendust_amd/csrc/od_issue_probe.hip)")
     ap.add_argument("--sweep-full", action="store_true",
                     help="only: the whole 64-simulation, 5000-step sweep end to end, batched and sequential")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only: the 64-simulation sweep leg (--sweep-steps, --sweep-seq-steps), its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -378,6 +382,9 @@ def main():
         return
     if args.issue_probe:
         issue_probe()
+        return
+    if args.sweep_only:
+        print(json.dumps(sweep_leg(args.sweep_steps, args.sweep_seq_steps)))
         return
 
     from opendust_amd import engine, workloads
@@ -400,7 +407,7 @@ def main():
         chunk, tile = eng.tune(2)
         log("autotuner chose %d source blocks per work-group, kernel variant %d %s"
             % (chunk, tile, engine.sym_variants()[tile - 1]))
-    chunk, tile = eng.sym_choice()
+    chunk, tile, bands = eng.sym_schedule()
 
     if args.pair_only:
         for _ in range(args.pair_only):
@@ -507,8 +514,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            # profiled per (variant, schedule); the untuned default is variant 1 with 16 source blocks per
-            # work-group at 2^20
+            # profiled per (variant, schedule); the untuned default at 2^20 is variant 6 with 4 source
+            # blocks per work-group
             "traffic": SYM_TRAFFIC_2E20.get((tile, chunk)) if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
@@ -517,7 +524,9 @@ def main():
             "pairs_per_launch": pairs_launch,
             "sym_chunk": chunk,
             "sym_tile": tile,
-            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd"), engine.sym_variants()[tile - 1]))
+            "sym_bands": bands,
+            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd", "vgpr_operands"),
+                                        engine.sym_variants()[tile - 1]))
                                if tile else None),
         },
         "stage_ms": stage_ms,

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define OD_ABI_VERSION 2
+#define OD_ABI_VERSION 3
 #define OD_MAX_DUST 8
 #define OD_MAX_TABLE 512
 #define OD_MAX_CELLS (1 << 27) /* cutoff mode: cells of the grid below */
@@ -86,10 +86,12 @@ typedef struct od_config {
   int32_t pair_mode;     /* OD_PAIR_*: which all-pairs kernel */
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
-  int32_t sym_tile;      /* symmetric kernel variant (tile shape and waves per SIMD), 1..od_sym_variants();
-                            0 = automatic (variant 1).  Cutoff mode, FP32c: the symmetric cell
-                            kernel's 64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4; 0 = 1 */
-  int32_t pad0_;
+  int32_t sym_tile;      /* symmetric kernel variant (tile shape, waves per SIMD, operand form),
+                            1..od_sym_variants(); 0 = automatic (variant 6 with at least 2048
+                            work-groups, else 3).  Cutoff mode, FP32c: the symmetric cell kernel's
+                            64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4; 0 = 1 */
+  int32_t sym_band_mib;  /* symmetric kernel: partials per band of rows of its triangular schedule, MiB;
+                            0 = 48 GB (larger schedules, about 2^21 ions and up, run in several bands) */
   uint64_t seed;         /* Philox4x32-10 key */
 
   /* geometry, integration (OpenMM units) */
@@ -228,16 +230,17 @@ int od_stage_timing(od_ctx* ctx, double* ms, int64_t* steps, int32_t reset);
  * symmetric kernel.  No counterpart in the reference (measurement support). */
 int od_pair_info(od_ctx* ctx, int32_t* mode, int64_t* pairs_per_launch);
 
-/* The symmetric kernel's current schedule: source blocks per work-group and
- * kernel variant (1-based), as configured, defaulted or tuned; 0 and 0 when
- * the context does not run it.  Cutoff mode, FP32c: 0 and the symmetric cell
- * kernel's variant (1, 2, 3: 1, 2, 4 target sub-blocks per wave).
- * Measurement support. */
-int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile);
+/* The symmetric kernel's current schedule: source blocks per work-group,
+ * kernel variant (1-based), as configured, defaulted or tuned, and the number
+ * of bands of rows its triangular schedule runs in (od_config.sym_band_mib);
+ * 0, 0 and 0 when the context does not run it.  Cutoff mode, FP32c: 0, the
+ * symmetric cell kernel's variant (1, 2, 3: 1, 2, 4 target sub-blocks per
+ * wave) and 0.  Measurement support. */
+int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* bands);
 
 /* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
  * return *sym_chunk = *sym_tile = 0 and change nothing; cutoff mode, FP32c:
- * it times the symmetric cell kernel's variants on the current ions, keeps
+ * it times the symmetric cell kernel's three variants on the current ions, keeps
  * the fastest and returns *sym_chunk = 0, *sym_tile = the variant).  It times the whole
  * pair evaluation `reps` times per candidate on the current ions: first 1, 2,
  * 4, 8, 16 and 32 source blocks per work-group with the configured kernel
@@ -252,10 +255,11 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile);
 int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile);
 
 /* Number of symmetric kernel variants, and the tile shape (64-target
- * sub-blocks per wave) and waves per SIMD of variant v (1-based) as
- * compiled.  Measurement support. */
+ * sub-blocks per wave), waves per SIMD and operand form (1: the pair step's
+ * full-rate instructions take VGPR operands only, 0: DPP/SGPR operands) of
+ * variant v (1-based) as compiled.  Measurement support. */
 int od_sym_variants(void);
-int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd);
+int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd, int32_t* vgpr_operands);
 
 /* Cutoff mode only: the cell list of the last force evaluation.  cell_of_ion[n]
  * (binned ions: their cell; ions left out: the number of cells), sorted_ions

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "od_device.h"
+#include "od_sym_tile.h"
 
 using namespace od;
 
@@ -223,6 +224,13 @@ __global__ __launch_bounds__(256) void k_load_pos(const double* __restrict__ X, 
 #include "od_kernels_trap.inc"
 #include "od_kernels_cell.inc"
 
+// Partial memory of one band of the symmetric schedule of a single context
+// (default of od_config.sym_band_mib): rows are grouped into consecutive bands
+// of at most this many bytes of partials, launched and reduced one after
+// another (one band, the usual case, is one launch and one reduction).
+// Batches run one band.
+constexpr double SYM_BAND_BYTES = 48e9;
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -260,12 +268,15 @@ struct od_ctx {
   int kst[KMARKS] = {};
   double stage_ms[OD_N_STAGES] = {};
   int64_t stage_steps = 0;
-  // symmetric pair kernel
+  // symmetric pair kernel; bands of rows: band b = rows [band_row[b],
+  // band_row[b+1]), its work-groups at launch-table and slot offset band_slot[b]
   int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0, sym_var = 0;
+  double band_cap = SYM_BAND_BYTES;
   int2* wg_tab = nullptr;
   int* wg_base = nullptr;
-  double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr;
+  double *sym_tp = nullptr, *sym_dg = nullptr, *rowD = nullptr, *sym_acc = nullptr;
   float* sym_sp = nullptr;
+  std::vector<int> band_row, band_slot;
   // cutoff mode: cell list; Gcut: flagged sources' share of the drift-state force;
   // sc_tp, sc_sp: partials of the symmetric cell kernel (FP32c)
   double* Gcut = nullptr;
@@ -274,7 +285,7 @@ struct od_ctx {
   int *scan_tsum = nullptr, *scan_toff = nullptr;
   int cut = 0, n_cells = 0, n_cols = 0, rs_nblk = 0, max_items = 0, rs_passes = 2;
   int nx = 0, ny = 0, nz = 0;  // cell grid
-  int sc_mt = 1;  // symmetric cell kernel: 64-target sub-blocks per wave (1, 2 or 4)
+  int sc_var = 0;  // symmetric cell kernel variant (index into cell_variant_mt)
   uint32_t *ckey[2] = {nullptr, nullptr}, *cval[2] = {nullptr, nullptr};
   uint32_t *skey = nullptr, *sval = nullptr;  // sorted keys/values: the buffers the last radix pass wrote
   int *rs_hist = nullptr, *rs_off = nullptr, *rs_total = nullptr, *cell_start = nullptr,
@@ -392,7 +403,7 @@ static void free_ctx(od_ctx* c) {
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
-                  c->rowD, c->sym_sp, c->Gcut, c->scan_tsum, c->scan_toff, c->sc_tp, c->sc_sp, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
+                  c->rowD, c->sym_sp, c->sym_acc, c->Gcut, c->scan_tsum, c->scan_toff, c->sc_tp, c->sc_sp, c->ckey[0], c->ckey[1], c->cval[0], c->cval[1], c->rs_hist,
                   c->rs_off, c->rs_total, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
@@ -441,12 +452,15 @@ static double sym_bytes(const std::vector<int>& be, int ch) {
   return (double)sym_wg_count(be, ch) * 3 * SYM_TB * (4.0 * ch + 8.0);
 }
 
+
 // (Re)builds a context's schedule for `ch` source blocks per work-group:
-// partial slots in row-major (X, k) order (wg_base), launch order longest
-// work-group first (ties in row-major order), so the short work-groups that
-// end each row fill the tail of the launch.  C: od_ctx or od_batch.
+// partial slots in row-major (X, k) order (wg_base); rows grouped into bands
+// of at most band_cap bytes of partials; within each band the launch order is
+// longest work-group first (ties in row-major order), so the short
+// work-groups that end each row fill the tail of the launch.  C: od_ctx or
+// od_batch.
 template <class C>
-static int sym_build(C* c, const std::vector<int>& be, int ch) {
+static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap) {
   const int NB = (int)be.size();
   const int64_t n_wg = sym_wg_count(be, ch);
   if (n_wg > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "symmetric pair kernel: too many workgroups");
@@ -463,11 +477,26 @@ static int sym_build(C* c, const std::vector<int>& be, int ch) {
     }
   }
   base[NB] = (int)tab.size();
-  std::vector<int> order(tab.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len[a] > len[b]; });
-  std::vector<int2> launch(tab.size());
-  for (size_t i = 0; i < order.size(); ++i) launch[i] = tab[order[i]];
+  // bands of rows: each holds at least one row and at most band_cap bytes of
+  // partials unless a single row exceeds it
+  const double wg_bytes = 3.0 * SYM_TB * (4.0 * ch + 8.0);
+  std::vector<int> band_row{0};
+  for (int X = 0; X < NB; ++X) {
+    const double rows_bytes = (double)(base[X + 1] - base[band_row.back()]) * wg_bytes;
+    if (X > band_row.back() && rows_bytes > band_cap) band_row.push_back(X);
+  }
+  band_row.push_back(NB);
+  std::vector<int2> launch;
+  launch.reserve(tab.size());
+  int max_band = 1;
+  for (size_t b = 0; b + 1 < band_row.size(); ++b) {
+    const int w0 = base[band_row[b]], w1 = base[band_row[b + 1]];
+    std::vector<int> order(w1 - w0);
+    for (int i = 0; i < w1 - w0; ++i) order[i] = w0 + i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int q) { return len[a] > len[q]; });
+    for (int i : order) launch.push_back(tab[i]);
+    max_band = std::max(max_band, w1 - w0);
+  }
   void* old[] = {c->wg_tab, c->wg_base, c->sym_tp, c->sym_sp};
   for (void* q : old)
     if (q) hipFree(q);
@@ -475,11 +504,14 @@ static int sym_build(C* c, const std::vector<int>& be, int ch) {
   c->wg_base = nullptr;
   c->sym_tp = nullptr;
   c->sym_sp = nullptr;
-  const size_t nw = std::max<size_t>(launch.size(), 1);
+  const size_t nw = std::max<size_t>(launch.size(), 1), nb_w = (size_t)max_band;
   int rc = OD_OK;
   if ((rc = dalloc(&c->wg_tab, nw * sizeof(int2))) || (rc = dalloc(&c->wg_base, base.size() * sizeof(int))) ||
-      (rc = dalloc(&c->sym_tp, nw * 3 * SYM_TB * 8)) || (rc = dalloc(&c->sym_sp, nw * 3 * ch * SYM_TB * 4)))
+      (rc = dalloc(&c->sym_tp, nb_w * 3 * SYM_TB * 8)) || (rc = dalloc(&c->sym_sp, nb_w * 3 * ch * SYM_TB * 4)))
     return rc;
+  c->band_row = band_row;
+  c->band_slot.assign(band_row.size(), 0);
+  for (size_t b = 0; b < band_row.size(); ++b) c->band_slot[b] = base[band_row[b]];
   if (!launch.empty()) HIPCHK(hipMemcpy(c->wg_tab, launch.data(), launch.size() * sizeof(int2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->wg_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
   c->n_blocks = NB;
@@ -489,20 +521,24 @@ static int sym_build(C* c, const std::vector<int>& be, int ch) {
 }
 
 // default kernel variant: the configured one, else 8 target sub-blocks per
-// wave (2 waves per work-group) when the schedule has at least 2048
-// work-groups (4 waves per SIMD), else 4 (4 waves per work-group: at 2^15 ions
-// the 496 work-groups then still give about 2 waves per SIMD).  Measured on
-// MI355X at 2^15 ions: 0.33 ms per evaluation with 4, 0.64 ms with 8.
+// wave (2 waves per work-group) with VGPR operands when the schedule has at
+// least 2048 work-groups (4 waves per SIMD), else 4 (4 waves per work-group:
+// at 2^15 ions the 496 work-groups then still give about 2 waves per SIMD).
+// Measured on MI355X at 2^15 ions: 0.33 ms per evaluation with 4, 0.64 ms with
+// 8; at 2^20, 4 source blocks per work-group: 308.5 ms with the DPP/SGPR
+// operand step, 287.7 ms with the VGPR operand step.
 static int sym_default_variant(int64_t n_wg, int configured) {
   if (configured > 0) return configured - 1;
-  return n_wg >= 2048 ? 0 : 2;
+  return n_wg >= 2048 ? 5 : 2;
 }
 
-// default source blocks per work-group: the largest of 16, 8, .. that still
-// gives >= 4096 work-groups (8 per XCD CU set), or the configured value
+// default source blocks per work-group: the largest of 4, 2, 1 that still
+// gives >= 4096 work-groups (8 per XCD CU set), or the configured value.
+// Measured at 2^20 with the default variant: 291.1 ms with 16, 287.7 with 4,
+// 289.7 with 2.
 static int sym_default_chunk(const std::vector<int>& be, int configured) {
   if (configured > 0) return configured;
-  int ch = 16;
+  int ch = 4;
   while (ch > 1 && sym_wg_count(be, ch) < 4096) ch /= 2;
   return ch;
 }
@@ -543,8 +579,8 @@ static int setup_cells(od_ctx* c, const od_config* cfg, const DevParams& hp) {
     if ((rc = dalloc(&c->sc_tp, (size_t)CC_NSET * 3 * np * 8)) ||
         (rc = dalloc(&c->sc_sp, (size_t)CC_NSET * 3 * np * 4)))
       return rc;
-    c->sc_mt = cell_default_mt(cfg->sym_tile);
-    if (c->sc_mt < 0) return set_err(OD_ERR_ARG, "cutoff mode: sym_tile %d outside [0, %d]", cfg->sym_tile, CC_N_VARIANTS);
+    c->sc_var = cell_variant_of(cfg->sym_tile);
+    if (c->sc_var < 0) return set_err(OD_ERR_ARG, "cutoff mode: sym_tile %d outside [0, %d]", cfg->sym_tile, CC_N_VARIANTS);
   }
   return OD_OK;
 }
@@ -559,9 +595,11 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   if (T < 2 || (!forced && c->n_pad < (1 << 15))) return OD_OK;
   const std::vector<int> be(T, T);
   const int ch = sym_default_chunk(be, cfg->sym_chunk);
-  if (!forced && sym_bytes(be, ch) > 48e9) return OD_OK;
+  c->band_cap = cfg->sym_band_mib > 0 ? cfg->sym_band_mib * 1048576.0 : SYM_BAND_BYTES;
   int rc = OD_OK;
-  if ((rc = sym_build(c, be, ch)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8))) return rc;
+  if ((rc = sym_build(c, be, ch, c->band_cap)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)) ||
+      (rc = dalloc(&c->sym_acc, (size_t)3 * c->n_pad * 8)))
+    return rc;
   c->sym_var = sym_default_variant(c->n_wg, cfg->sym_tile);
   c->sym = 1;
   return OD_OK;
@@ -683,15 +721,35 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
     const float a = (float)(c->cfg.r_0 / c->cfg.r_D_e * L), linv = (float)(1.0 / L);
     if (timed) HIPCHK(hipEventRecord(c->ev0, c->stream));
     const SymVariant& V = sym_variants[c->sym_var];
-    hipLaunchKernelGGL(V.fn, dim3(c->n_wg), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
-                       c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, a, linv, (const int*)nullptr,
-                       (const float*)nullptr, c->sym_tp, c->sym_sp);
-    if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+    const int nbands = (int)c->band_row.size() - 1;
+    if (nbands == 1) {
+      hipLaunchKernelGGL(V.fn, dim3(c->n_wg), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
+                         c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, 0, a, linv, (const int*)nullptr,
+                         (const float*)nullptr, c->sym_tp, c->sym_sp);
+      if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
+      hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
+                         (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
+      hipLaunchKernelGGL(k_sym_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->sym_dg, c->sym_tp,
+                         c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, (const int*)nullptr,
+                         (const double*)nullptr, F);
+      return OD_OK;
+    }
+    // bands of rows, each launched and reduced into acc in order (the timed
+    // region then holds the band reductions between the launches)
     hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
                        (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
-    hipLaunchKernelGGL(k_sym_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->sym_dg, c->sym_tp,
-                       c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, (const int*)nullptr,
-                       (const double*)nullptr, F);
+    for (int b = 0; b < nbands; ++b) {
+      const int w0 = c->band_slot[b], nw = c->band_slot[b + 1] - w0;
+      if (nw > 0)
+        hipLaunchKernelGGL(V.fn, dim3(nw), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
+                           c->sym_ch, (const int2*)(c->wg_tab + w0), (const int*)c->wg_base, w0, a, linv,
+                           (const int*)nullptr, (const float*)nullptr, c->sym_tp, c->sym_sp);
+      if (timed && b == nbands - 1) HIPCHK(hipEventRecord(c->ev1, c->stream));
+      hipLaunchKernelGGL(k_sym_band_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, (const double*)c->sym_dg,
+                         (const double*)c->sym_tp, (const float*)c->sym_sp, (const int*)c->wg_base, c->band_row[b],
+                         c->band_row[b + 1], w0, c->sym_ch, c->n_pad, -C * L * L, b == 0 ? 1 : 0,
+                         b == nbands - 1 ? 1 : 0, c->sym_acc, F);
+    }
     return OD_OK;
   }
   double* out = (c->n_chunks > 1) ? c->Fpart : F;
@@ -774,11 +832,11 @@ static int launch_forces_t(od_ctx* c, const double* X, const void* pos, const ui
     // symmetric: forward neighbour columns and the own column, per-ion reduction
     const float4* sp4 = (const float4*)c->spos;
     const dim3 g(c->n_cols * CC_NSET);
-    const int ki = cell_variant_index(c->sc_mt) * 2 + (sw ? 1 : 0);
+    const int ki = c->sc_var * 2 + (sw ? 1 : 0);
     hipLaunchKernelGGL(cell_sym_kernels[ki], g, dim3(64), 0, c->stream, c->dP, sp4, c->skey, c->cell_start, c->sc_tp,
                        c->sc_sp);
     hipLaunchKernelGGL(k_symcol_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->dP, c->skey, c->sval,
-                       c->cell_start, 64 * c->sc_mt, (const double*)c->sc_tp, (const float*)c->sc_sp, F);
+                       c->cell_start, 64 * cell_variant_mt[c->sc_var], (const double*)c->sc_tp, (const float*)c->sc_sp, F);
   } else if (sw) {
     hipLaunchKernelGGL((k_pair_cell<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->dP, (const PV*)c->spos,
                        c->skey, c->sval, c->cell_start, c->items, c->n_items, F);
@@ -1128,7 +1186,7 @@ static int64_t cell_candidates(const od_ctx* c, const std::vector<int>& cs, cons
     if (a0 == a1) continue;
     const int ax = A % nx, ay = A / nx;
     if (c->prec == OD_PREC_FP32C) {
-      const int G = 64 * c->sc_mt;
+      const int G = 64 * cell_variant_mt[c->sc_var];
       for (int k = 0; k < CC_NSET; ++k) {
         const bool own = k == CC_NFWD;
         int dx = 0, dy = 0;
@@ -1198,16 +1256,18 @@ extern "C" int od_cutoff_pair_count(od_ctx* c, int64_t* in_range, int64_t* candi
   return OD_OK;
 }
 
-extern "C" int od_sym_choice(od_ctx* c, int32_t* sym_chunk, int32_t* sym_tile) {
+extern "C" int od_sym_choice(od_ctx* c, int32_t* sym_chunk, int32_t* sym_tile, int32_t* bands) {
   if (!c) return set_err(OD_ERR_ARG, "null context");
   if (c->cut && c->prec == OD_PREC_FP32C) {
-    // symmetric cell kernel: no source blocks, the variant (MT 1, 2, 4 -> 1, 2, 3)
+    // symmetric cell kernel: no source blocks, the variant 1 .. CC_N_VARIANTS
     if (sym_chunk) *sym_chunk = 0;
-    if (sym_tile) *sym_tile = cell_variant_index(c->sc_mt) + 1;
+    if (sym_tile) *sym_tile = c->sc_var + 1;
+    if (bands) *bands = 0;
     return OD_OK;
   }
   if (sym_chunk) *sym_chunk = c->sym ? c->sym_ch : 0;
   if (sym_tile) *sym_tile = c->sym ? c->sym_var + 1 : 0;
+  if (bands) *bands = c->sym ? (int32_t)c->band_row.size() - 1 : 0;
   return OD_OK;
 }
 
@@ -1386,8 +1446,12 @@ extern "C" const char* od_build_info(void) { return "opendust_amd engine, gfx950
 // and can be fixed through od_config.sym_chunk and od_config.sym_tile.
 static const int kTuneChunks[] = {1, 2, 4, 8, 16, 32};
 
+// band_cap: the band size schedules are built with (batches: unlimited, one
+// band).  Candidates whose partials exceed skip_cap in all (48 GB) are
+// skipped, which keeps tuning short; larger schedules keep their default.
 template <class C, class F>
-static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen_ch, int* chosen_var) {
+static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, int* chosen_ch, int* chosen_var,
+                    double band_cap, double skip_cap) {
   double best_ms = 1e300;
   int best_ch = c->sym_ch, best_var = c->sym_var;
   // a captured step holds the schedule's buffers, which sym_build replaces:
@@ -1398,8 +1462,8 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
   c->graph = nullptr;
   c->graph_steps = 0;
   auto time_one = [&](int ch, int var) -> int {
-    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > 48e9) return OD_OK;
-    if (c->sym_ch != ch) CHKRET(sym_build(c, be, ch));
+    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > skip_cap) return OD_OK;
+    if (c->sym_ch != ch) CHKRET(sym_build(c, be, ch, band_cap));
     c->sym_var = var;
     evaluate();  // warm-up
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1425,7 +1489,7 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
     for (int ch : kTuneChunks)
       if (ch != ch1 && (ch == 2 * ch1 || 2 * ch == ch1)) CHKRET(time_one(ch, best_var));
   }
-  if (c->sym_ch != best_ch) CHKRET(sym_build(c, be, best_ch));
+  if (c->sym_ch != best_ch) CHKRET(sym_build(c, be, best_ch, band_cap));
   c->sym_var = best_var;
   *chosen_ch = best_ch;
   *chosen_var = best_var;
@@ -1434,16 +1498,18 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
 
 extern "C" int od_sym_variants(void) { return SYM_N_VARIANTS; }
 
-extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd) {
+extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd,
+                                   int32_t* vgpr_operands) {
   if (v < 1 || v > SYM_N_VARIANTS) return set_err(OD_ERR_ARG, "variant %d outside [1, %d]", v, SYM_N_VARIANTS);
   if (targets_per_lane) *targets_per_lane = sym_variants[v - 1].mt;
   if (waves_per_simd) *waves_per_simd = sym_variants[v - 1].wpe;
+  if (vgpr_operands) *vgpr_operands = sym_variants[v - 1].vgpr_step;
   return OD_OK;
 }
 
 // Autotuner of the symmetric cell kernel (cutoff mode, FP32c): times `reps`
 // whole force evaluations (binning included) on the current positions for
-// each variant (MT 1, 2, 4) and keeps the fastest.
+// each variant (cell_sym_kernels) and keeps the fastest.
 static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
   if (c->gexec) hipGraphExecDestroy(c->gexec);
   if (c->graph) hipGraphDestroy(c->graph);
@@ -1454,9 +1520,9 @@ static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
   CHKRET(dalloc(&Fs, (size_t)3 * c->n_pad * 8));
   int rc = load_positions(c, c->X, c->pi, c->pj);
   double best_ms = 1e300;
-  int best = c->sc_mt;
+  int best = c->sc_var;
   for (int v = 0; v < CC_N_VARIANTS && rc == OD_OK; ++v) {
-    c->sc_mt = cell_variant_mt[v];
+    c->sc_var = v;
     rc = launch_forces(c, c->X, c->pj, nullptr, Fs, false);  // warm-up
     if (rc == OD_OK && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = set_err(OD_ERR_HIP, "hipEventRecord");
     for (int r = 0; r < reps && rc == OD_OK; ++r) rc = launch_forces(c, c->X, c->pj, nullptr, Fs, false);
@@ -1466,12 +1532,12 @@ static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
       rc = set_err(OD_ERR_HIP, "timing the cell kernel variants failed");
     if (rc == OD_OK && ms < best_ms) {
       best_ms = ms;
-      best = c->sc_mt;
+      best = v;
     }
   }
   hipFree(Fs);
-  c->sc_mt = best;
-  *chosen_var = cell_variant_index(best);
+  c->sc_var = best;
+  *chosen_var = best;
   return rc;
 }
 
@@ -1494,7 +1560,8 @@ extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk, int32_t* sym
     double* Fs = nullptr;
     CHKRET(dalloc(&Fs, (size_t)3 * c->n_pad * 8));
     const std::vector<int> be(c->n_blocks, c->n_blocks);
-    const int rc = tune_sym(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen, &var);
+    const int rc = tune_sym(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen, &var,
+                            c->band_cap, SYM_BAND_BYTES);
     hipFree(Fs);
     if (rc != OD_OK) return rc;
     // the ion-ion force of the current positions, with the chosen schedule
@@ -1519,7 +1586,8 @@ extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk, int3
                          (float4*)b->pi, (float4*)b->pj);
     double* Fs = nullptr;
     CHKRET(dalloc(&Fs, (size_t)3 * b->n_tot * 8));
-    const int rc = tune_sym(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen, &var);
+    const int rc = tune_sym(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen, &var,
+                            1e300, SYM_BAND_BYTES);
     hipFree(Fs);
     if (rc != OD_OK) return rc;
     b->stale = true;  // the next run recomputes the forces with the chosen schedule

@@ -10,9 +10,20 @@
 //                sqrt, exp2, fma + rcp, fma, 2 mul, 3 fma (accumulation) per
 //                pair; law<0> replaces the three transcendentals by full-rate
 //                ops, so the two times give their cost in that mix.
+//   form<F>      16 register-only instructions of one encoding per iteration
+//                (8 chains), written in asm so the compiler cannot change the
+//                form: the issue cost of VGPR, SGPR, DPP and transcendental
+//                operands (cycles per wave64 instruction per SIMD).
+//   sym<VG>      the symmetric kernels' register tile (od_sym_tile.h) alone:
+//                8 target sub-blocks per lane against a 64-source sub-block
+//                per iteration, no memory traffic; VG = 1 the VGPR-operand
+//                step, 0 the DPP/SGPR-operand step.  Its pair rate is the
+//                ceiling of k_pair32_sym's inner loop.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+
+#include "od_sym_tile.h"
 
 template <int NF, int NT, int OP>
 __global__ __launch_bounds__(256) void probe_mix(float* out, int iters, float a) {
@@ -81,6 +92,75 @@ __global__ __launch_bounds__(256) void probe_law(float* out, int iters, float a,
   out[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+#define OD_FORM16(I)                                                                          \
+  asm volatile(I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7) I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7) \
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)  \
+               : "v"(b), "v"(c), "s"(sb))
+#define OD_FMAC_V(k) "v_fmac_f32_e32 %" #k ", %8, %9\n"
+#define OD_FMA_V(k) "v_fma_f32 %" #k ", %" #k ", %8, %9\n"
+#define OD_MUL_V(k) "v_mul_f32_e32 %" #k ", %8, %" #k "\n"
+#define OD_SUB_DPP(k) "v_sub_f32_dpp %" #k ", %8, %" #k " row_ror:1 row_mask:0xf bank_mask:0xf\n"
+#define OD_EXP(k) "v_exp_f32_e64 %" #k ", -%" #k "\n"
+#define OD_FMA_S(k) "v_fma_f32 %" #k ", %" #k ", %10, 0.5\n"
+#define OD_FMAC_S(k) "v_fmac_f32_e32 %" #k ", %10, %9\n"
+#define OD_MIX(k) "v_fmac_f32_e32 %" #k ", %8, %9\n"
+// 12 fmac + 4 exp: chains 3 and 7 take the exponentials
+#define OD_FORM_MIX                                                                                          \
+  asm volatile(OD_FMAC_V(0) OD_FMAC_V(1) OD_FMAC_V(2) OD_EXP(3) OD_FMAC_V(4) OD_FMAC_V(5) OD_FMAC_V(6) OD_EXP(7) \
+                   OD_FMAC_V(0) OD_FMAC_V(1) OD_FMAC_V(2) OD_EXP(3) OD_FMAC_V(4) OD_FMAC_V(5) OD_FMAC_V(6)   \
+                       OD_EXP(7)                                                                             \
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)            \
+               : "v"(b), "v"(c), "s"(sb))
+
+template <int F>
+__global__ __launch_bounds__(256) void probe_form(float* out, int iters, float sb) {
+  float a0 = 0.1f + threadIdx.x * 1e-3f, a1 = a0 + 0.01f, a2 = a0 + 0.02f, a3 = a0 + 0.03f, a4 = a0 + 0.04f,
+        a5 = a0 + 0.05f, a6 = a0 + 0.06f, a7 = a0 + 0.07f;
+  const float b = 0.9999f, c = 0.25f;
+  for (int i = 0; i < iters; ++i) {
+    if constexpr (F == 0) OD_FORM16(OD_FMAC_V);   // v_fmac_f32_e32, VGPR operands
+    if constexpr (F == 1) OD_FORM16(OD_FMA_V);    // v_fma_f32 (VOP3), VGPR operands
+    if constexpr (F == 2) OD_FORM16(OD_MUL_V);    // v_mul_f32_e32, VGPR operands
+    if constexpr (F == 3) OD_FORM16(OD_SUB_DPP);  // v_sub_f32 with a DPP row rotation
+    if constexpr (F == 4) OD_FORM16(OD_EXP);      // v_exp_f32
+    if constexpr (F == 5) OD_FORM16(OD_FMA_S);    // v_fma_f32 with an SGPR and an inline constant
+    if constexpr (F == 6) OD_FORM_MIX;            // 12 v_fmac_f32 (VGPR) + 4 v_exp_f32
+    if constexpr (F == 7) OD_FORM16(OD_FMAC_S);   // v_fmac_f32_e32 with an SGPR operand
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
+template <int VG>
+__global__ __launch_bounds__(128) void probe_sym(float* out, int iters, float a, float linv) {
+  using namespace od;
+  SymT<8> T;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    T.x[m] = 0.01f * (threadIdx.x + 64 * m);
+    T.y[m] = -0.007f * (threadIdx.x + 32 * m);
+    T.z[m] = 0.005f * (threadIdx.x + 16 * m);
+    T.ax[m] = T.ay[m] = T.az[m] = 0.f;
+  }
+  const SymLaw L = VG ? SymLaw{vgpr(a), vgpr(linv), 0.f, 0.f, 0.f} : SymLaw{a, linv, 0.f, 0.f, 0.f};
+  const int lane = threadIdx.x & 63;
+  float xj = 0.3f + 1e-3f * lane, yj = -0.2f + 2e-3f * lane, zj = 0.1f - 1e-3f * lane;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (int i = 0; i < iters; ++i) {
+    float bx = 0.f, by = 0.f, bz = 0.f;
+    sym_tile<8, 0, VG>(T, xj, yj, zj, bx, by, bz, L);
+    sx += bx;
+    sy += by;
+    sz += bz;
+    xj += 1e-4f;
+    yj -= 2e-4f;
+    zj += 3e-4f;
+  }
+  float s = sx + sy + sz;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) s += T.ax[m] + T.ay[m] + T.az[m];
+  out[blockIdx.x * 128 + threadIdx.x] = s;
+}
+
 template <class K>
 static double time_ms(K launch, int reps) {
   hipEvent_t e0, e1;
@@ -105,6 +185,7 @@ int main() {
   const int iters = 20000, reps = 5;
   const double waves = (double)blocks * 4;
   float* d = nullptr;
+  // output: one float per thread of the largest launch (form and mix: blocks x 256; sym: 16 x CUs x 128)
   if (hipMalloc(&d, (size_t)blocks * 256 * sizeof(float)) != hipSuccess) return 1;
   printf("{\"device\": \"%s\", \"cus\": %d, \"clock_khz\": %d, \"waves_per_simd\": 8, \"iters\": %d,\n", p.gcnArchName,
          p.multiProcessorCount, p.clockRate, iters);
@@ -131,8 +212,40 @@ int main() {
             hipLaunchKernelGGL(probe_law<0>, dim3(blocks), dim3(256), 0, 0, d, iters, 1.44e-3f, 0.6931f);
         },
         reps);
-    printf(" \"law_%s\": {\"ms\": %.4f, \"pairs_per_s\": %.5e}%s\n", tr ? "trans" : "fullrate", ms,
-           waves * 64 * iters * 4 / (ms * 1e-3), tr ? "" : ",");
+    printf(" \"law_%s\": {\"ms\": %.4f, \"pairs_per_s\": %.5e},\n", tr ? "trans" : "fullrate", ms,
+           waves * 64 * iters * 4 / (ms * 1e-3));
+  }
+  // instruction forms, 8 waves per SIMD
+  const double simd_hz = 4.0 * p.multiProcessorCount * p.clockRate * 1e3;
+  auto form = [&](const char* name, auto kern) {
+    const double ms =
+        time_ms([&]() { hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, d, iters, 0.9999f); }, reps);
+    const double rate = waves * iters * 16 / (ms * 1e-3);
+    printf(" \"%s\": {\"ms\": %.4f, \"wave_instr_per_s\": %.5e, \"cycles_per_wave_instr_per_simd\": %.3f},\n",
+           name, ms, rate, simd_hz / rate);
+  };
+  form("form_fmac_vgpr", probe_form<0>);
+  form("form_fma_vop3_vgpr", probe_form<1>);
+  form("form_mul_vgpr", probe_form<2>);
+  form("form_sub_dpp", probe_form<3>);
+  form("form_exp", probe_form<4>);
+  form("form_fma_sgpr_inline", probe_form<5>);
+  form("form_12fmac_4exp", probe_form<6>);
+  form("form_fmac_sgpr", probe_form<7>);
+  // the symmetric register tile: 2-wave blocks, as many as the registers allow
+  const int sym_blocks = p.multiProcessorCount * 16, sym_iters = 100;
+  for (int vg = 0; vg < 2; ++vg) {
+    const double ms = time_ms(
+        [&]() {
+          if (vg)
+            hipLaunchKernelGGL(probe_sym<1>, dim3(sym_blocks), dim3(128), 0, 0, d, sym_iters, 0.05f, 0.6931f);
+          else
+            hipLaunchKernelGGL(probe_sym<0>, dim3(sym_blocks), dim3(128), 0, 0, d, sym_iters, 0.05f, 0.6931f);
+        },
+        reps);
+    const double unordered = (double)sym_blocks * 2 * sym_iters * 64.0 * 8 * 64;
+    printf(" \"sym_tile_%s\": {\"ms\": %.4f, \"unordered_pairs_per_s\": %.5e, \"ordered_pairs_per_s\": %.5e}%s\n",
+           vg ? "vgpr" : "dpp_sgpr", ms, unordered / (ms * 1e-3), 2 * unordered / (ms * 1e-3), vg ? "" : ",");
   }
   printf("}\n");
   (void)hipFree(d);

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libopendust_amd.so")
 
 MAX_DUST = host.MAX_DUST
 MAX_TABLE = host.MAX_TABLE
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 RUN_GRAPH = 1
 RUN_PROFILE_PAIR = 2
@@ -75,7 +75,7 @@ class Config(ctypes.Structure):
         ("sym_chunk", ctypes.c_int32),
         ("sim_id", ctypes.c_int32),
         ("sym_tile", ctypes.c_int32),
-        ("pad0_", ctypes.c_int32),
+        ("sym_band_mib", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("R", ctypes.c_double), ("H", ctypes.c_double), ("dt", ctypes.c_double),
         ("r_D_e", ctypes.c_double), ("r_0", ctypes.c_double),
@@ -197,8 +197,8 @@ def load_library(path=LIB_PATH):
     lib.od_batch_destroy.argtypes = [c_ctx]
     lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_sym_variants.argtypes = []
-    lib.od_sym_choice.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32)]
-    lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
+    lib.od_sym_choice.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]
+    lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_update_config.argtypes = [c_ctx, P(Config)]
     lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
     lib.od_write_csv.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
@@ -239,14 +239,15 @@ def check_abi():
 
 
 def sym_variants():
-    """[(64-target sub-blocks per wave, waves per SIMD)] of the symmetric kernel
-    variants; od_config.sym_tile = index + 1 selects one."""
+    """[(64-target sub-blocks per wave, waves per SIMD, VGPR operands 0/1)] of
+    the symmetric kernel variants; od_config.sym_tile = index + 1 selects one."""
     lib = load_library()
     out = []
     for v in range(1, lib.od_sym_variants() + 1):
-        mt, wps = ctypes.c_int32(), ctypes.c_int32()
-        _check(lib.od_sym_variant_info(v, ctypes.byref(mt), ctypes.byref(wps)), "od_sym_variant_info")
-        out.append((mt.value, wps.value))
+        mt, wps, vg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.od_sym_variant_info(v, ctypes.byref(mt), ctypes.byref(wps), ctypes.byref(vg)),
+               "od_sym_variant_info")
+        out.append((mt.value, wps.value, vg.value))
     return out
 
 
@@ -457,9 +458,16 @@ class Engine:
     def sym_choice(self):
         """(source blocks per work-group, kernel variant) the symmetric kernel
         runs with now; (0, 0) when the context does not run it."""
-        ch, tile = ctypes.c_int32(), ctypes.c_int32()
-        _check(self.lib.od_sym_choice(self._ctx, ctypes.byref(ch), ctypes.byref(tile)), "od_sym_choice")
-        return ch.value, tile.value
+        return self.sym_schedule()[:2]
+
+    def sym_schedule(self):
+        """(source blocks per work-group, kernel variant, bands of rows) of the
+        symmetric kernel; zeros when the context does not run it (cutoff
+        mode: (0, cell kernel variant, 0))."""
+        ch, tile, bands = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.od_sym_choice(self._ctx, ctypes.byref(ch), ctypes.byref(tile), ctypes.byref(bands)),
+               "od_sym_choice")
+        return ch.value, tile.value, bands.value
 
     def pair_forces(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

@@ -97,7 +97,7 @@ def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
     """Every variant of the symmetric kernel the autotuner may pick (tile
     shape, waves per SIMD) meets the FP32c force bound against the oracle, for
@@ -125,6 +125,66 @@ def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
     bad.sym_tile = len(variants) + 1
     with pytest.raises(RuntimeError, match="sym_tile"):
         eng.Engine(bad)
+
+
+def test_sym_schedule_in_bands(oracle_lib):
+    """The symmetric kernel's triangular schedule run in bands of rows (each
+    launched and reduced in order into an FP64 accumulator; the default band
+    is 48 GB of partials, here 1 MiB, so every band holds one or two rows):
+    forces within the FP32c bound of the oracle and within 1e-6 of the
+    one-band schedule (only the summation order differs; the engine reports
+    one band, then several), and a run is bitwise reproducible, eager and
+    graph-replayed.
+
+    At 20000 ions in the cylinder close pairs are common, so the rounding of
+    the positions to FP32 dominates the error against exact positions; the
+    kernel's arithmetic is checked against the oracle on the rounded
+    positions (per ion 1e-5 |F_i| + 1e-6 rms|F|, compensated sums) and the
+    total against exact positions only up to what that rounding alone moves
+    (FP64 on rounded positions vs exact, + 5e-6)."""
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=20000, precision="fp32c", craft=False, mode="sym")
+    f_orc = oracle_lib.pair_forces(cfg, x0)
+    f_rnd = oracle_lib.pair_forces(cfg, oracle_lib.fp32c_rounded_positions(cfg, x0))
+    fn = np.linalg.norm(f_rnd, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    rel_q = np.linalg.norm(f_rnd - f_orc) / np.linalg.norm(f_orc)
+    forces = {}
+    for mib in (0, 1):
+        cfg.sym_band_mib = mib
+        say("sym schedule, band MiB", mib, "N", cfg.n_ions)
+        e = eng.Engine(cfg)
+        assert e.pair_info()[0] == MODES["sym"]
+        bands = e.sym_schedule()[2]
+        say("  bands of rows", bands)
+        assert (bands == 1) if mib == 0 else (bands > 2)
+        forces[mib] = e.pair_forces(x0)
+        e.close()
+        err = np.linalg.norm(forces[mib] - f_rnd, axis=1)
+        rel_a = np.linalg.norm(forces[mib] - f_rnd) / np.linalg.norm(f_rnd)
+        rel_b = np.linalg.norm(forces[mib] - f_orc) / np.linalg.norm(f_orc)
+        say("  arithmetic: norm-wise", rel_a, "worst per-ion err / bound", np.max(err / (1e-5 * fn + 1e-6 * rms)),
+            "| vs exact positions", rel_b, "(rounding alone", rel_q, ")")
+        assert rel_a <= 2e-6
+        assert np.all(err <= 1e-5 * fn + 1e-6 * rms)
+        assert rel_b <= rel_q + 5e-6
+    # the FP64 sums of a few FP32-precision partials per ion are usually exact,
+    # so the two orders mostly agree bit for bit
+    d = np.linalg.norm(forces[1] - forces[0]) / np.linalg.norm(forces[0])
+    say("  banded vs one band: norm-wise", d)
+    assert d <= 1e-6
+    # steps with ions crafted into the dust and beyond the walls
+    od, cfg, x0, v0 = _setup(N=20000, precision="fp32c", craft=True, mode="sym")
+    cfg.sym_band_mib = 1
+    outs = []
+    for flags in (0, 0, eng.RUN_GRAPH):
+        e = eng.Engine(cfg)
+        e.set_ions(x0, v0)
+        rec = e.run(8, flags)
+        outs.append((rec, e.get_ions()[0]))
+        e.close()
+    for rec, x in outs[1:]:
+        assert np.array_equal(rec, outs[0][0]) and np.array_equal(x, outs[0][1])
 
 
 def test_full_size_sym_forces_vs_fp64_gpu():
@@ -481,6 +541,39 @@ def test_bottom_face_reinjection_vs_oracle(oracle_lib, precision, thresholds):
     say("  records, error / tolerance:", ratios)
     for key, r in ratios.items():
         assert r <= 1.0, key
+
+
+@pytest.mark.parametrize("cut", [False, True])
+def test_stage_timing_covers_the_step(cut):
+    """od_stage_timing: every stage of a profiled step is timed (non-negative,
+    the cell-list stage only with a cutoff), the stage times add up to the
+    step's, and profiling changes no result (bitwise against an unprofiled run)."""
+    import time
+
+    eng = _engine_mod()
+    od, cfg, x0, v0 = _setup(N=8192, precision="fp32c", craft=True)
+    if cut:
+        cfg.cutoff = 1.5 * cfg.r_D_e
+    say("stage timing, cutoff", cut, "N", cfg.n_ions)
+    outs = []
+    for flags in (eng.RUN_PROFILE_STAGES, 0):
+        e = eng.Engine(cfg)
+        e.set_ions(x0, v0)
+        t0 = time.perf_counter()
+        rec = e.run(4, flags)
+        wall = (time.perf_counter() - t0) / 4 * 1e3
+        if flags:
+            st, n = e.stage_timing(reset=True)
+            assert n == 4 and e.stage_timing()[1] == 0
+        outs.append((rec, e.get_ions()[0]))
+        e.close()
+    say("  ms per step:", {k: round(v, 4) for k, v in st.items()}, "wall", round(wall, 3))
+    assert all(v >= 0.0 for v in st.values())
+    assert (st["cells"] > 0.0) == cut and (st["corr"] > 0.0) == cut
+    for k in ("kick_drift", "list", "pair", "kick2", "inflow", "dust"):
+        assert st[k] > 0.0, k
+    assert sum(st.values()) <= 4.0 * wall + 1.0
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
 def test_trap_grid_vs_oracle(oracle_lib):

@@ -117,6 +117,41 @@ def test_cutoff_dense_cells_vs_oracle(oracle_lib, precision, rc, rs):
 
 
 @pytest.mark.timeout(600)
+@pytest.mark.parametrize("rc,rs", [(2.0, None), (3.2, 3.0)])
+def test_cutoff_cell_variants_vs_oracle(oracle_lib, rc, rs):
+    """Every variant of the symmetric cell kernel the cutoff autotuner may
+    pick (1, 2 or 4 target sub-blocks per wave), hard and switched law, at
+    the dense workload's density: FP32c bound against the oracle on the
+    rounded positions, each variant bitwise reproducible."""
+    eng = _eng()
+    from oracle.oracle import fp32c_rounded_positions
+
+    od, cfg, x0, v0 = _dense_setup("fp32c", rc, rs)
+    xr = fp32c_rounded_positions(cfg, x0)
+    f_r = oracle_lib.pair_forces(cfg, xr)
+    edge = oracle_lib.cut_edge_count(cfg, xr, 1e-6)
+    f_cut = oracle_lib.cut_force_at_cutoff(cfg)
+    flip = np.sqrt(np.sum((edge * f_cut) ** 2))
+    tiles = []
+    for tile in range(1, 4):
+        cfg.sym_tile = tile
+        say("dense cutoff, cell kernel variant", tile, "rc", rc, "rs", rs)
+        e = eng.Engine(cfg)
+        f1 = e.pair_forces(x0)
+        f2 = e.pair_forces(x0)
+        tiles.append(e.sym_choice()[1])
+        e.close()
+        assert np.array_equal(f1, f2)
+        _check_rows(f1, f_r, edge, f_cut, 1e-5, 1e-6, "variant %d vs oracle on rounded positions" % tile)
+        assert np.linalg.norm(f1 - f_r) <= 2e-6 * np.linalg.norm(f_r) + flip
+    assert tiles == [1, 2, 3]
+    bad = eng.Config.from_buffer_copy(cfg)
+    bad.sym_tile = 4
+    with pytest.raises(RuntimeError, match="sym_tile"):
+        eng.Engine(bad)
+
+
+@pytest.mark.timeout(600)
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_cutoff_many_cells_vs_oracle(oracle_lib, precision):
     """A grid of 40 x 40 x 80 = 128000 cells (R = 10, H = 40 r_D_e, cutoff 1
@@ -266,6 +301,55 @@ def test_cutoff_full_size_fp32c_vs_fp64_and_oracle_rows(oracle_lib):
     # the close pairs a larger share of each force
     assert rel_q <= rel_round + 5e-6
     assert rel_round <= 3e-4
+
+
+@pytest.mark.timeout(900)
+def test_all_pairs_2e22_symmetric_in_bands_vs_fp64():
+    """All-pairs at 2^22 ions (the cutoff workload's cylinder, no cutoff): the
+    symmetric FP32c kernel's schedule holds 155 GB of partials at its default
+    4 source blocks per work-group, so it runs in four bands of at most 48
+    GB, each launched and reduced in order.  Against the FP64 row kernel: on
+    the FP32-rounded positions norm-wise <= 2e-6 and per ion <= 2e-5 |F_i| +
+    2e-6 rms|F|, the 2^20 bound scaled by sqrt(N / 2^20) for the random walk
+    of the per-term roundings over N sources (every block pair of every band
+    lands); on the exact positions the FP64 kernel's own rounded-minus-exact
+    difference to 5e-6."""
+    from opendust_amd import workloads
+    from oracle.oracle import fp32c_rounded_positions
+
+    eng = _eng()
+    od = _quiet(workloads.cutoff_large, N=2 ** 22, n=2, precision="fp64", seed=8, cutoff=0.0)
+    od.set_cutoff(False)
+    alpha = workloads.synthetic_alpha(od)
+    cfg, x0, v0 = _quiet(workloads.prepare, od, alpha=alpha, np_seed=9, precision="fp64")
+    assert cfg.cutoff == 0.0
+    xr = fp32c_rounded_positions(cfg, x0)
+    say("2^22 all-pairs: fp64 rows N", cfg.n_ions)
+    e = eng.Engine(cfg)
+    f64 = e.pair_forces(x0)
+    f64r = e.pair_forces(xr)
+    e.close()
+    cfg32 = eng.Config.from_buffer_copy(cfg)
+    cfg32.precision = 0
+    cfg32.pair_mode = 2
+    say("2^22 all-pairs: fp32c symmetric, in bands")
+    e = eng.Engine(cfg32)
+    sched = e.sym_schedule()
+    say("  schedule (source blocks per work-group, variant, bands)", sched)
+    assert e.pair_info()[0] == 2 and sched[0] == 4 and sched[2] >= 4
+    f32 = e.pair_forces(x0)
+    e.close()
+    fn = np.linalg.norm(f64r, axis=1)
+    rms = np.sqrt(np.mean(fn ** 2))
+    err = np.linalg.norm(f32 - f64r, axis=1)
+    rel_a = np.linalg.norm(f32 - f64r) / np.linalg.norm(f64r)
+    rel_q = np.linalg.norm(f32 - f64) / np.linalg.norm(f64)
+    rel_round = np.linalg.norm(f64r - f64) / np.linalg.norm(f64)
+    say("  arithmetic: norm-wise", rel_a, "worst per-ion err / bound", np.max(err / (2e-5 * fn + 2e-6 * rms)),
+        "| vs exact positions", rel_q, "(rounding alone", rel_round, ")")
+    assert rel_a <= 2e-6
+    assert np.all(err <= 2e-5 * fn + 2e-6 * rms)
+    assert rel_q <= rel_round + 5e-6
 
 
 @pytest.mark.timeout(600)
