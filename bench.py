@@ -39,7 +39,23 @@ SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes pe
     (2, 1): 43.531e9, (2, 2): 30.513e9, (2, 4): 23.763e9, (2, 8): 20.361e9, (2, 16): 18.394e9, (2, 32): 17.445e9,
     (3, 1): 27.449e9, (3, 2): 20.549e9, (3, 4): 17.092e9, (3, 8): 15.010e9, (3, 16): 13.793e9, (3, 32): 12.922e9,
     (4, 1): 31.020e9, (4, 2): 22.758e9, (4, 4): 18.547e9, (4, 8): 16.588e9, (4, 16): 15.480e9, (4, 32): 14.710e9,
+    (6, 4): 16.871e9,
 }
+# variants whose registers spill (scratch traffic of their own); for the others
+# the schedule fixes the traffic (partials and positions): variants 1 and 3
+# agree within 2.6 % at every profiled block count, 1 and 6 within 0.2 % at 4
+SYM_SPILLING_VARIANTS = (2, 4)
+
+
+def sym_traffic_2e20(tile, chunk):
+    """(bytes per launch, source) of k_pair32_sym at 2^20 ions: the PMC
+    reading of this (variant, block count), else that of variant 1 with the
+    same block count when neither spills, else (None, None)."""
+    if (tile, chunk) in SYM_TRAFFIC_2E20:
+        return SYM_TRAFFIC_2E20[(tile, chunk)], "pmc"
+    if tile not in SYM_SPILLING_VARIANTS and (1, chunk) in SYM_TRAFFIC_2E20:
+        return SYM_TRAFFIC_2E20[(1, chunk)], "pmc of variant 1 with the same schedule (neither spills)"
+    return None, None
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 # Algorithmic HBM bytes per ion of the per-ion stages of a step (FP32c): what
@@ -305,8 +321,15 @@ def sweep_full(n_sims=64, sweep_len=5000, seq_steps=200, out_path=None):
     np.random.seed(21)
     log("sweep end to end: batched, %d steps" % sweep_len)
     t0 = time.perf_counter()
+    last = [t0]
+
+    def progress(done, total):
+        if time.perf_counter() - last[0] > 30.0 or done == total:
+            last[0] = time.perf_counter()
+            log("sweep end to end: batched, %d of %d steps, %.0f s" % (done, total, last[0] - t0))
+
     with contextlib.redirect_stdout(io.StringIO()):
-        sweep.simulateBatch(ods, "0", useGraph=True)
+        sweep.simulateBatch(ods, "0", useGraph=True, progress=progress)
     t_batch = time.perf_counter() - t0
     log("sweep end to end: batched %.1f s" % t_batch)
     ods = make()
@@ -438,6 +461,8 @@ def main():
     value = pairs_step * args.steps / wall
     kernel_ms = pair_ms / max(launches, 1)
     achieved = pairs_launch * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12
+    traffic, traffic_src = (sym_traffic_2e20(tile, chunk) if (mode == engine.PAIR_SYM and N == 2 ** 20)
+                            else (None, None))
     log("fp32c: %.2f ms/step, pair kernel %.2f ms, %.3e pairs/s" % (ms_per_step, kernel_ms, value))
 
     # FP64 GPU path and FP32c-vs-FP64 dust force error from the same state
@@ -516,7 +541,8 @@ def main():
             "frac": achieved / FP32_PEAK_TFLOPS,
             # profiled per (variant, schedule); the untuned default at 2^20 is variant 6 with 4 source
             # blocks per work-group
-            "traffic": SYM_TRAFFIC_2E20.get((tile, chunk)) if (mode == engine.PAIR_SYM and N == 2 ** 20) else None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "traffic_unit": "bytes per launch",
             "kernel": "k_pair32_sym" if mode == engine.PAIR_SYM else "k_pair32<2>",
             "kernel_ms": kernel_ms,

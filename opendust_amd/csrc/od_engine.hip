@@ -397,9 +397,9 @@ static int fill_params(const od_config* c, int n_pad, DevParams* P) {
 
 static void free_ctx(od_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->dev);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  (void)hipSetDevice(c->dev);  // teardown: nothing to report a failure to
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->dP, c->X, c->V, c->F, c->Fpart, c->q, c->partials, c->rowF, c->pi, c->pj,
                   c->list_old, c->list_new, c->flags, c->st, c->face, c->block_cnt, c->block_off, c->n_flag,
                   c->slot_of, c->list_ion, c->err, c->rec, c->wg_tab, c->wg_base, c->sym_tp, c->sym_dg,
@@ -407,12 +407,12 @@ static void free_ctx(od_ctx* c) {
                   c->rs_off, c->rs_total, c->cell_start, c->cell_chunks, c->item_off, c->items,
                   c->n_items, c->spos};
   for (void* b : bufs)
-    if (b) hipFree(b);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
+    if (b) (void)hipFree(b);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipEvent_t e : c->kev)
-    if (e) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -499,7 +499,7 @@ static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap) 
   }
   void* old[] = {c->wg_tab, c->wg_base, c->sym_tp, c->sym_sp};
   for (void* q : old)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   c->wg_tab = nullptr;
   c->wg_base = nullptr;
   c->sym_tp = nullptr;
@@ -1010,8 +1010,8 @@ extern "C" int od_update_config(od_ctx* c, const od_config* cfg) {
   c->cfg.sym_chunk = keep_chunk;
   c->cfg.sym_tile = keep_tile;
   // captured steps hold kernel arguments derived from the old parameters
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
   c->graph_steps = 0;
@@ -1052,11 +1052,11 @@ extern "C" int od_set_step(od_ctx* c, int64_t step) {
 static int build_graph(od_ctx* c, int steps) {
   if (c->gexec && c->graph_steps == steps) return OD_OK;
   if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
+    (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
   }
   if (c->graph) {
-    hipGraphDestroy(c->graph);
+    (void)hipGraphDestroy(c->graph);
     c->graph = nullptr;
   }
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1249,7 +1249,7 @@ extern "C" int od_cutoff_pair_count(od_ctx* c, int64_t* in_range, int64_t* candi
   if (e == hipSuccess) e = hipMemcpyAsync(cs.data(), c->cell_start, cs.size() * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(key.data(), c->skey, key.size() * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_cutoff_pair_count: %s", hipGetErrorString(e));
   if (in_range) *in_range = (int64_t)h;
   if (candidates) *candidates = cell_candidates(c, cs, key);
@@ -1339,10 +1339,10 @@ extern "C" int od_pair_forces(od_ctx* c, const double* x, double* f) {
   int rc = OD_OK;
   if ((rc = dalloc(&dX, 3 * np * 8)) || (rc = dalloc(&dF, 3 * np * 8)) || (rc = dalloc(&pi, np * c->pos_bytes)) ||
       (rc = dalloc(&pj, np * c->pos_bytes))) {
-    hipFree(dX);
-    hipFree(dF);
-    hipFree(pi);
-    hipFree(pj);
+    (void)hipFree(dX);
+    (void)hipFree(dF);
+    (void)hipFree(pi);
+    (void)hipFree(pj);
     return rc;
   }
   std::vector<double> sx;
@@ -1356,10 +1356,10 @@ extern "C" int od_pair_forces(od_ctx* c, const double* x, double* f) {
     if (e == hipSuccess) e = hipMemcpyAsync(sf.data(), dF, sf.size() * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   }
-  hipFree(dX);
-  hipFree(dF);
-  hipFree(pi);
-  hipFree(pj);
+  (void)hipFree(dX);
+  (void)hipFree(dF);
+  (void)hipFree(pi);
+  (void)hipFree(pj);
   if (e != hipSuccess) return set_err(OD_ERR_HIP, "od_pair_forces: %s", hipGetErrorString(e));
   if (rc != OD_OK) return rc;
   for (int i = 0; i < c->n; ++i)
@@ -1385,13 +1385,13 @@ extern "C" int od_trap_potential_grid(int32_t device, double R, double H, int32_
          *dout = nullptr;
   int rc = OD_OK;
   auto cleanup = [&]() {
-    hipFree(xs);
-    hipFree(ys);
-    hipFree(zs);
-    hipFree(drho);
-    hipFree(dh);
-    hipFree(part);
-    hipFree(dout);
+    (void)hipFree(xs);
+    (void)hipFree(ys);
+    (void)hipFree(zs);
+    (void)hipFree(drho);
+    (void)hipFree(dh);
+    (void)hipFree(part);
+    (void)hipFree(dout);
   };
   if ((rc = dalloc(&xs, n_mc * 8)) || (rc = dalloc(&ys, n_mc * 8)) || (rc = dalloc(&zs, n_mc * 8)) ||
       (rc = dalloc(&drho, n_r * 8)) || (rc = dalloc(&dh, n_h * 8)) ||
@@ -1456,8 +1456,8 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
   int best_ch = c->sym_ch, best_var = c->sym_var;
   // a captured step holds the schedule's buffers, which sym_build replaces:
   // drop it before the first rebuild, so no error path can leave it behind
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
   c->graph_steps = 0;
@@ -1511,8 +1511,8 @@ extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t
 // whole force evaluations (binning included) on the current positions for
 // each variant (cell_sym_kernels) and keeps the fastest.
 static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
   c->graph_steps = 0;
@@ -1535,7 +1535,7 @@ static int tune_cells(od_ctx* c, int reps, int* chosen_var) {
       best = v;
     }
   }
-  hipFree(Fs);
+  (void)hipFree(Fs);
   c->sc_var = best;
   *chosen_var = best;
   return rc;
@@ -1562,7 +1562,7 @@ extern "C" int od_tune(od_ctx* c, int32_t reps, int32_t* sym_chunk, int32_t* sym
     const std::vector<int> be(c->n_blocks, c->n_blocks);
     const int rc = tune_sym(c, be, reps, [&]() { launch_pair(c, c->pj, c->pj, Fs, false); }, &chosen, &var,
                             c->band_cap, SYM_BAND_BYTES);
-    hipFree(Fs);
+    (void)hipFree(Fs);
     if (rc != OD_OK) return rc;
     // the ion-ion force of the current positions, with the chosen schedule
     CHKRET(load_positions(c, c->X, c->pi, c->pj));
@@ -1588,7 +1588,7 @@ extern "C" int od_batch_tune(od_batch* b, int32_t reps, int32_t* sym_chunk, int3
     CHKRET(dalloc(&Fs, (size_t)3 * b->n_tot * 8));
     const int rc = tune_sym(b, b->blk_end_host, reps, [&]() { batch_pair<float>(b, false, Fs); }, &chosen, &var,
                             1e300, SYM_BAND_BYTES);
-    hipFree(Fs);
+    (void)hipFree(Fs);
     if (rc != OD_OK) return rc;
     b->stale = true;  // the next run recomputes the forces with the chosen schedule
   }
@@ -1657,7 +1657,7 @@ static int density_run(hipStream_t st, const double* z_dev, int64_t N, int n, do
   double *d_zg = nullptr, *d_s = nullptr;
   int rc = OD_OK;
   if ((rc = dalloc(&d_zg, (size_t)n * 8)) || (rc = dalloc(&d_s, (size_t)n * 8))) {
-    if (d_zg) hipFree(d_zg);
+    if (d_zg) (void)hipFree(d_zg);
     return rc;
   }
   std::vector<double> sums(n);
@@ -1668,8 +1668,8 @@ static int density_run(hipStream_t st, const double* z_dev, int64_t N, int n, do
   }
   if (e == hipSuccess) e = hipMemcpyAsync(sums.data(), d_s, (size_t)n * 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  hipFree(d_zg);
-  hipFree(d_s);
+  (void)hipFree(d_zg);
+  (void)hipFree(d_s);
   if (e != hipSuccess) return set_err(OD_ERR_HIP, "density profile: %s", hipGetErrorString(e));
   density_finish(n, R, L, N_s, sums, z_grid, conc);
   return OD_OK;
@@ -1763,6 +1763,6 @@ extern "C" int od_density_profile_z(int32_t device, const double* z_nm, int64_t 
   if (N > 0 && hipMemcpy(d_z, z_nm, (size_t)N * 8, hipMemcpyHostToDevice) != hipSuccess)
     rc = set_err(OD_ERR_HIP, "density profile: upload failed");
   if (rc == OD_OK) rc = density_run(nullptr, d_z, N, n_nodes, R, L, N_s, z_grid, conc);
-  hipFree(d_z);
+  (void)hipFree(d_z);
   return rc;
 }

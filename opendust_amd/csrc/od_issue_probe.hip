@@ -185,10 +185,10 @@ int main() {
   const int iters = 20000, reps = 5;
   const double waves = (double)blocks * 4;
   float* d = nullptr;
-  // output: one float per thread of the largest launch (form and mix: blocks x 256; sym: 16 x CUs x 128)
+  // output: one float per thread of the largest launch (form and mix: blocks x 256; sym: at most 12 x CUs x 128)
   if (hipMalloc(&d, (size_t)blocks * 256 * sizeof(float)) != hipSuccess) return 1;
-  printf("{\"device\": \"%s\", \"cus\": %d, \"clock_khz\": %d, \"waves_per_simd\": 8, \"iters\": %d,\n", p.gcnArchName,
-         p.multiProcessorCount, p.clockRate, iters);
+  printf("{\"device\": \"%s\", \"cus\": %d, \"clock_khz\": %d, \"lds_per_cu\": %zu, \"waves_per_simd\": 8, \"iters\": %d,\n",
+         p.gcnArchName, p.multiProcessorCount, p.clockRate, (size_t)p.maxSharedMemoryPerMultiProcessor, iters);
   auto mix = [&](const char* name, auto kern, int n_instr, bool last = false) {
     const double ms = time_ms([&]() { hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, d, iters, 0.999f); },
                               reps);
@@ -232,20 +232,31 @@ int main() {
   form("form_fma_sgpr_inline", probe_form<5>);
   form("form_12fmac_4exp", probe_form<6>);
   form("form_fmac_sgpr", probe_form<7>);
-  // the symmetric register tile: 2-wave blocks, as many as the registers allow
-  const int sym_blocks = p.multiProcessorCount * 16, sym_iters = 100;
+  // the symmetric register tile: 2-wave blocks, exactly 2 wps blocks per CU
+  // (wps waves per SIMD): each block reserves LDS it does not use, so that no
+  // more than 2 wps blocks fit in a CU's LDS, and the grid holds 2 wps blocks
+  // per CU (both forms use at most 80 VGPRs, so up to 6 waves per SIMD fit)
+  const int sym_iters = 100;
   for (int vg = 0; vg < 2; ++vg) {
-    const double ms = time_ms(
-        [&]() {
-          if (vg)
-            hipLaunchKernelGGL(probe_sym<1>, dim3(sym_blocks), dim3(128), 0, 0, d, sym_iters, 0.05f, 0.6931f);
-          else
-            hipLaunchKernelGGL(probe_sym<0>, dim3(sym_blocks), dim3(128), 0, 0, d, sym_iters, 0.05f, 0.6931f);
-        },
-        reps);
-    const double unordered = (double)sym_blocks * 2 * sym_iters * 64.0 * 8 * 64;
-    printf(" \"sym_tile_%s\": {\"ms\": %.4f, \"unordered_pairs_per_s\": %.5e, \"ordered_pairs_per_s\": %.5e}%s\n",
-           vg ? "vgpr" : "dpp_sgpr", ms, unordered / (ms * 1e-3), 2 * unordered / (ms * 1e-3), vg ? "" : ",");
+    printf(" \"sym_tile_%s\": {", vg ? "vgpr" : "dpp_sgpr");
+    const int wps_list[3] = {3, 4, 6};
+    for (int w = 0; w < 3; ++w) {
+      const int per_cu = 2 * wps_list[w];
+      const int sym_blocks = p.multiProcessorCount * per_cu;
+      const size_t lds = ((size_t)p.maxSharedMemoryPerMultiProcessor / per_cu) & ~(size_t)1023;
+      const double ms = time_ms(
+          [&]() {
+            if (vg)
+              hipLaunchKernelGGL(probe_sym<1>, dim3(sym_blocks), dim3(128), lds, 0, d, sym_iters, 0.05f, 0.6931f);
+            else
+              hipLaunchKernelGGL(probe_sym<0>, dim3(sym_blocks), dim3(128), lds, 0, d, sym_iters, 0.05f, 0.6931f);
+          },
+          reps);
+      const double unordered = (double)sym_blocks * 2 * sym_iters * 64.0 * 8 * 64;
+      printf("\"%d_waves_per_simd\": {\"ms\": %.4f, \"unordered_pairs_per_s\": %.5e, \"lds_per_block\": %zu}%s",
+             wps_list[w], ms, unordered / (ms * 1e-3), lds, w < 2 ? ", " : "");
+    }
+    printf("}%s\n", vg ? "" : ",");
   }
   printf("}\n");
   (void)hipFree(d);

@@ -180,13 +180,14 @@ __device__ __forceinline__ void sym_step(SymT<MT>& T, float xj, float yj, float 
   }
 }
 
-// The same pair step with every full-rate operand in a VGPR: on gfx950 FP32
-// VALU instructions whose operands are all VGPRs issue at about 2.5 cycles per
-// wave64 instruction per SIMD, those with an SGPR, literal or DPP operand at
-// about 4.75 (measured: `bench.py --issue-probe`).  The source coordinates are
-// rotated by one lane per step with the reactions (three v_mov_b32_dpp shared
-// by the MT target sub-blocks) instead of riding on the subtractions, and the
-// law's constants live in VGPRs (SymLaw filled through vgpr()).
+// The same pair step with every full-rate operand in a VGPR: the source
+// coordinates are rotated by one lane per step with the reactions (three more
+// v_mov_b32_dpp, shared by the MT target sub-blocks) instead of riding on the
+// subtractions, and the law's constants live in VGPRs (SymLaw filled through
+// vgpr()).  Free of the DPP asm blocks, the compiler fits 8 target sub-blocks
+// in 127 VGPRs (4 waves per SIMD) where the DPP form takes 145 (3 waves per
+// SIMD); at equal occupancy the two forms run at the same rate (the register
+// tile alone, `bench.py --issue-probe`), so occupancy is what makes it faster.
 template <int MT, int CUT>
 __device__ __forceinline__ void sym_step_v(SymT<MT>& T, float& xs, float& ys, float& zs, float& bx, float& by,
                                            float& bz, const SymLaw& L, float eps) {
@@ -269,7 +270,8 @@ __device__ __forceinline__ void sym_phase_v(SymT<MT>& T, float& xs, float& ys, f
 // The 64 x 64 pairs of the wave's source sub-block (xj, yj, zj, one per lane)
 // with its MT target sub-blocks: four phases of 16 rotation steps, rows
 // exchanging sources between phases.  On return (bx, by, bz) hold the
-// reaction on the lane's own source and T.a* the targets' accumulators.
+// reaction on the lane's own source (added to what they held) and T.a* the
+// targets' accumulators.
 // VG: the VGPR-operand step (sym_step_v; L's constants must be VGPRs).
 template <int MT, int CUT, int VG = 0>
 __device__ __forceinline__ void sym_tile(SymT<MT>& T, float xj, float yj, float zj, float& bx, float& by, float& bz,
