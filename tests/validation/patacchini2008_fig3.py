@@ -19,7 +19,10 @@ orbit and collection z-forces, each with its blocking standard error
 between the averaging window's two halves (a drift check), and
 z = (ours - reference) / (sqrt(2) sigma).
 
-    python tests/validation/patacchini2008_fig3.py --points 0 [--steps 100000] [--out results.json]
+    python tests/validation/patacchini2008_fig3.py --points 0 [--steps 100000] [--realisation 1] [--out results.json]
+
+--realisation R > 0 runs an independent realisation (Philox key, trap seed and
+NumPy seed offset by 1000 R); R = 0 is the default run.
 
 Runs on the GPU (one point of 100000 steps: about 9 minutes on one MI355X).
 """
@@ -73,22 +76,24 @@ def main():
     ap.add_argument("--points", default="0")
     ap.add_argument("--steps", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=8)
+    ap.add_argument("--realisation", type=int, default=0)
     args = ap.parse_args()
     ref = np.loadtxt(os.path.join(os.path.dirname(HERE), "golden", "patacchini2008_fig3_reference.txt"))
     results = []
     for p in (int(t) for t in args.points.split(",")):
         with contextlib.redirect_stdout(io.StringIO()):
-            od = build(p, args.steps, args.seed + p)
+            od = build(p, args.steps, args.seed + p + 1000 * args.realisation)
         od.outputParameters.nOutput = 10000
         od.outputParameters.nFileOutput = 10000
-        np.random.seed(300 + p)
+        np.random.seed(300 + p + 1000 * args.realisation)
         t0 = time.time()
         print("w_c %.4e, E %.4e V/m: %d steps" % (W_C[p], E_FIELD[p], args.steps), flush=True)
         od.simulate("0")
         wall = time.time() - t0
         d = od.dustParticles[0]
         start = args.steps - min(50000, args.steps // 2) + 1
-        row = {"w_c": W_C[p], "steps": args.steps, "average_from_step": start, "wall_s": wall}
+        row = {"w_c": W_C[p], "steps": args.steps, "realisation": args.realisation, "average_from_step": start,
+               "wall_s": wall}
         for name, series, r in (("q", d.q, ref[p, 1]), ("F_orbit_z", d.forceIonsOrbit[:, 2], ref[p, 2]),
                                 ("F_collect_z", d.forceIonsCollect[:, 2], ref[p, 3])):
             a = np.asarray(series[start:], dtype=np.float64)
