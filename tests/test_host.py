@@ -284,6 +284,10 @@ def test_validation_fixtures_match_the_scan_tables():
         return mod
 
     golden = os.path.join(here, "golden")
+    f5 = np.loadtxt(os.path.join(golden, "piel2017_fig5_reference.txt"))
+    assert f5.shape == (12, 3) and np.allclose(f5[:, 0], script("piel2017_fig5").MACH, rtol=0, atol=1e-5)
+    # the orbit force rises to a maximum near M = 0.8 and falls again, as the reference printed it
+    assert np.argmax(f5[:, 1]) == 6 and np.all(f5[:, 1] > 0)
     f6 = np.loadtxt(os.path.join(golden, "piel2017_fig6_reference.txt"))
     assert f6.shape == (7, 5) and np.allclose(f6[:, 0], script("piel2017_fig6").MACH, rtol=0, atol=1e-5)
     f7 = np.loadtxt(os.path.join(golden, "piel2017_fig7_reference.txt"))
@@ -294,6 +298,25 @@ def test_validation_fixtures_match_the_scan_tables():
     # the field gives Mach 1 at every collision frequency (v_fl = e E / (m_i w_c))
     assert np.allclose(np.array(s.E_FIELD) / np.array(s.W_C), s.E_FIELD[0] / s.W_C[0], rtol=1e-6)
     assert np.all(pt[:, 1] < 0) and np.all(np.abs(pt[:, 3]) > 0)
+
+
+def test_bench_traffic_lookup_labels_its_source():
+    """The bench's HBM traffic of k_pair32_sym at 2^20 ions: the PMC reading
+    of a profiled (variant, block count); for an unprofiled pair, that of
+    variant 1 with the same block count, labelled, unless the variant spills
+    registers; otherwise none."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.sym_traffic_2e20(6, 4) == (bench.SYM_TRAFFIC_2E20[(6, 4)], "pmc")
+    assert bench.sym_traffic_2e20(2, 16) == (bench.SYM_TRAFFIC_2E20[(2, 16)], "pmc")
+    t, src = bench.sym_traffic_2e20(7, 8)
+    assert t == bench.SYM_TRAFFIC_2E20[(1, 8)] and "variant 1" in src
+    assert bench.sym_traffic_2e20(4, 64) == (None, None)
+    assert all(v not in bench.SYM_SPILLING_VARIANTS for v in (1, 3, 5, 6, 7, 8))
 
 
 def test_bench_refuses_more_than_one_gpu():
