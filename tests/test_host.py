@@ -313,7 +313,8 @@ def test_bench_traffic_lookup_labels_its_source():
     spec.loader.exec_module(bench)
     assert bench.sym_traffic_2e20(6, 4) == (bench.SYM_TRAFFIC_2E20[(6, 4)], "pmc")
     assert bench.sym_traffic_2e20(2, 16) == (bench.SYM_TRAFFIC_2E20[(2, 16)], "pmc")
-    t, src = bench.sym_traffic_2e20(7, 8)
+    assert (8, 8) not in bench.SYM_TRAFFIC_2E20
+    t, src = bench.sym_traffic_2e20(8, 8)
     assert t == bench.SYM_TRAFFIC_2E20[(1, 8)] and "variant 1" in src
     assert bench.sym_traffic_2e20(4, 64) == (None, None)
     assert all(v not in bench.SYM_SPILLING_VARIANTS for v in (1, 3, 5, 6, 7, 8))
