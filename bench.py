@@ -48,6 +48,16 @@ SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes pe
 SYM_SPILLING_VARIANTS = (2, 4)
 
 
+def pair_evaluations(n, sym, pairs_launch, n_flagged, n_absorbed_rows):
+    """Pair-term evaluations of one step (see the bench line's
+    kernel_pair_evaluations_per_step)."""
+    main = pairs_launch / 2.0 if sym else float(pairs_launch)
+    in_block = float(n) * (n - 1) - pairs_launch if sym else 0.0
+    corrections = 2.0 * (n - n_flagged) * n_flagged + float(n) * (n_flagged + n_absorbed_rows)
+    return {"main_launch": main, "in_block": in_block, "flagged_corrections": corrections,
+            "total": main + in_block + corrections, "per_interaction": (main + in_block + corrections) / (n * (n - 1.0))}
+
+
 def sym_traffic_2e20(tile, chunk):
     """(bytes per launch, source) of k_pair32_sym at 2^20 ions: the PMC
     reading of this (variant, block count), else that of variant 1 with the
@@ -453,7 +463,8 @@ def main():
     # per-stage HIP-event times of a few more steps (outside the timed region)
     eng.run(3, engine.RUN_PROFILE_STAGES)
     stage_ms, _ = eng.stage_timing()
-    n_flagged = eng.step_info()[2]
+    flags_last, _, n_flagged = eng.step_info()
+    n_absorbed_rows = int(np.count_nonzero(flags_last & ((1 << engine.MAX_DUST) - 1)))
     eng.close()
     log("stages (ms/step):", {k: round(v, 4) for k, v in stage_ms.items()})
 
@@ -559,6 +570,14 @@ def main():
         "stage_ms": stage_ms,
         "hbm_stages": hbm_stages(stage_ms, N, STAGE_BYTES_PER_ION),
         "flagged_ions_last_step": n_flagged,
+        # pair-term evaluations the kernels perform in a step (the metric counts
+        # N(N-1) interactions once per step): the main launch (one evaluation per
+        # unordered pair between blocks, symmetric kernel; one per ordered pair
+        # otherwise), the in-block pairs, and the flagged ions' corrections (their
+        # share as sources at the drifted and at the reinjected positions, and the
+        # full rows of the absorbed and of the reinjected ions), last step's counts
+        "kernel_pair_evaluations_per_step": pair_evaluations(N, mode == engine.PAIR_SYM, pairs_launch, n_flagged,
+                                                             n_absorbed_rows),
         "cpu_baseline": cpu,
         "fp64_gpu": fp64,
         "cutoff_2e22": cut,

@@ -320,6 +320,28 @@ def test_bench_traffic_lookup_labels_its_source():
     assert all(v not in bench.SYM_SPILLING_VARIANTS for v in (1, 3, 5, 6, 7, 8))
 
 
+def test_bench_pair_evaluation_count():
+    """The bench's count of pair-term evaluations per step: symmetric main
+    launch (one per unordered pair between blocks) + in-block ordered pairs +
+    the flagged ions' corrections, against a direct count for a small case."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    n, blk, nf, na = 3000, 1024, 7, 2
+    sizes = [min(blk, n - b) for b in range(0, n, blk)]
+    between = n * (n - 1) - sum(s * (s - 1) for s in sizes)  # ordered pairs in different blocks
+    ev = bench.pair_evaluations(n, True, between, nf, na)
+    assert ev["main_launch"] == between // 2
+    assert ev["in_block"] == sum(s * (s - 1) for s in sizes)
+    assert ev["flagged_corrections"] == 2 * (n - nf) * nf + n * (nf + na)
+    assert ev["total"] == ev["main_launch"] + ev["in_block"] + ev["flagged_corrections"]
+    rows = bench.pair_evaluations(n, False, n * (n - 1), nf, na)
+    assert rows["main_launch"] == n * (n - 1) and rows["in_block"] == 0
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU; --gpus N > 1 exits with a message before
     touching a device."""
