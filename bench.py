@@ -58,13 +58,14 @@ def pair_evaluations(n, sym, pairs_launch, n_flagged, n_absorbed_rows):
             "total": main + in_block + corrections, "per_interaction": (main + in_block + corrections) / (n * (n - 1.0))}
 
 
-def sym_traffic_2e20(tile, chunk):
+def sym_traffic_2e20(tile, chunk, rows=1):
     """(bytes per launch, source) of k_pair32_sym at 2^20 ions: the PMC
     reading of this (variant, block count), else that of variant 1 with the
-    same block count when neither spills, else (None, None)."""
+    same block count when neither spills and the variant does not group
+    target blocks (rows > 1 writes fewer partials), else (None, None)."""
     if (tile, chunk) in SYM_TRAFFIC_2E20:
         return SYM_TRAFFIC_2E20[(tile, chunk)], "pmc"
-    if tile not in SYM_SPILLING_VARIANTS and (1, chunk) in SYM_TRAFFIC_2E20:
+    if rows == 1 and tile not in SYM_SPILLING_VARIANTS and (1, chunk) in SYM_TRAFFIC_2E20:
         return SYM_TRAFFIC_2E20[(1, chunk)], "pmc of variant 1 with the same schedule (neither spills)"
     return None, None
 METRIC = "pair interactions/s + steps/s at 2^20 ions (FP32-comp vs FP64); sweep sims/hour"
@@ -473,8 +474,8 @@ def main():
     value = pairs_step * args.steps / wall
     kernel_ms = pair_ms / max(launches, 1)
     achieved = pairs_launch * FLOP_PER_PAIR / (kernel_ms * 1e-3) / 1e12
-    traffic, traffic_src = (sym_traffic_2e20(tile, chunk) if (mode == engine.PAIR_SYM and N == 2 ** 20)
-                            else (None, None))
+    traffic, traffic_src = (sym_traffic_2e20(tile, chunk, engine.sym_variants()[tile - 1][3])
+                            if (mode == engine.PAIR_SYM and N == 2 ** 20 and tile) else (None, None))
     log("fp32c: %.2f ms/step, pair kernel %.2f ms, %.3e pairs/s" % (ms_per_step, kernel_ms, value))
 
     # FP64 GPU path and FP32c-vs-FP64 dust force error from the same state
@@ -563,7 +564,8 @@ def main():
             "sym_chunk": chunk,
             "sym_tile": tile,
             "sym_bands": bands,
-            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd", "vgpr_operands"),
+            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd", "vgpr_operands",
+                                         "target_blocks_per_work_group"),
                                         engine.sym_variants()[tile - 1]))
                                if tile else None),
         },

@@ -255,11 +255,14 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* b
 int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile);
 
 /* Number of symmetric kernel variants, and the tile shape (64-target
- * sub-blocks per wave), waves per SIMD and operand form (1: the pair step's
- * full-rate instructions take VGPR operands only, 0: DPP/SGPR operands) of
- * variant v (1-based) as compiled.  Measurement support. */
+ * sub-blocks per wave), waves per SIMD, operand form (1: the pair step's
+ * full-rate instructions take VGPR operands only, 0: DPP/SGPR operands) and
+ * target blocks per work-group (> 1: row groups that share each source
+ * partial, one simulation only, not in batches) of variant v (1-based) as
+ * compiled.  Measurement support. */
 int od_sym_variants(void);
-int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd, int32_t* vgpr_operands);
+int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd, int32_t* vgpr_operands,
+                        int32_t* target_blocks);
 
 /* Cutoff mode only: the cell list of the last force evaluation.  cell_of_ion[n]
  * (binned ions: their cell; ions left out: the number of cells), sorted_ions

@@ -270,7 +270,7 @@ struct od_ctx {
   int64_t stage_steps = 0;
   // symmetric pair kernel; bands of rows: band b = rows [band_row[b],
   // band_row[b+1]), its work-groups at launch-table and slot offset band_slot[b]
-  int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0, sym_var = 0;
+  int sym = 0, n_blocks = 0, sym_ch = 0, n_wg = 0, sym_var = 0, sym_R = 1;  // sym_R: target blocks per row group
   double band_cap = SYM_BAND_BYTES;
   int2* wg_tab = nullptr;
   int* wg_base = nullptr;
@@ -442,50 +442,55 @@ static void pair_geometry(int n_pad, int ib, int j_split, int* n_chunks, int* ch
 // Work-groups of the triangular schedule: target block X, source blocks
 // X+1+k*ch .. X+(k+1)*ch clipped at be[X] (the first block past X's
 // simulation), k = 0, 1, ...
-static int64_t sym_wg_count(const std::vector<int>& be, int ch) {
+// work-groups of the schedule with `ch` source blocks per work-group and row
+// groups of R blocks (group g: rows g R ..; sources from its first row + 1)
+static int64_t sym_wg_count(const std::vector<int>& be, int ch, int R = 1) {
   int64_t n = 0;
-  for (int X = 0; X < (int)be.size(); ++X) n += (be[X] - 1 - X + ch - 1) / ch;
+  for (int X = 0; X < (int)be.size(); X += R) n += std::max(0, (be[X] - 1 - X + ch - 1) / ch);
   return n;
 }
 
-static double sym_bytes(const std::vector<int>& be, int ch) {
-  return (double)sym_wg_count(be, ch) * 3 * SYM_TB * (4.0 * ch + 8.0);
+static double sym_bytes(const std::vector<int>& be, int ch, int R = 1) {
+  return (double)sym_wg_count(be, ch, R) * 3 * SYM_TB * (4.0 * ch + 8.0 * R);
 }
 
 
-// (Re)builds a context's schedule for `ch` source blocks per work-group:
-// partial slots in row-major (X, k) order (wg_base); rows grouped into bands
-// of at most band_cap bytes of partials; within each band the launch order is
-// longest work-group first (ties in row-major order), so the short
-// work-groups that end each row fill the tail of the launch.  C: od_ctx or
-// od_batch.
+// (Re)builds a context's schedule for `ch` source blocks per work-group and
+// row groups of R target blocks (R > 1: one simulation only): partial slots in
+// group-major (g, k) order (wg_base, indexed by group); groups gathered into
+// bands of at most band_cap bytes of partials; within each band the launch
+// order is longest work-group first (ties in group-major order), so the
+// short work-groups that end each group's row fill the tail of the launch.
+// C: od_ctx or od_batch.
 template <class C>
-static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap) {
+static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap, int R) {
   const int NB = (int)be.size();
-  const int64_t n_wg = sym_wg_count(be, ch);
+  const int NG = (NB + R - 1) / R;
+  const int64_t n_wg = sym_wg_count(be, ch, R);
   if (n_wg > (1ll << 31) - 1) return set_err(OD_ERR_ARG, "symmetric pair kernel: too many workgroups");
   std::vector<int2> tab;
-  std::vector<int> base(NB + 1), len;
+  std::vector<int> base(NG + 1), len;
   tab.reserve(n_wg);
   len.reserve(n_wg);
-  for (int X = 0; X < NB; ++X) {
-    base[X] = (int)tab.size();
-    const int nk = (be[X] - 1 - X + ch - 1) / ch;
+  for (int g = 0; g < NG; ++g) {
+    const int X = g * R;
+    base[g] = (int)tab.size();
+    const int nk = std::max(0, (be[X] - 1 - X + ch - 1) / ch);
     for (int k = 0; k < nk; ++k) {
-      tab.push_back(make_int2(X, k));
+      tab.push_back(make_int2(g, k));
       len.push_back(std::min(ch, be[X] - (X + 1 + k * ch)));
     }
   }
-  base[NB] = (int)tab.size();
-  // bands of rows: each holds at least one row and at most band_cap bytes of
-  // partials unless a single row exceeds it
-  const double wg_bytes = 3.0 * SYM_TB * (4.0 * ch + 8.0);
+  base[NG] = (int)tab.size();
+  // bands of row groups: each holds at least one group and at most band_cap
+  // bytes of partials unless a single group exceeds it
+  const double wg_bytes = 3.0 * SYM_TB * (4.0 * ch + 8.0 * R);
   std::vector<int> band_row{0};
-  for (int X = 0; X < NB; ++X) {
-    const double rows_bytes = (double)(base[X + 1] - base[band_row.back()]) * wg_bytes;
-    if (X > band_row.back() && rows_bytes > band_cap) band_row.push_back(X);
+  for (int g = 0; g < NG; ++g) {
+    const double rows_bytes = (double)(base[g + 1] - base[band_row.back()]) * wg_bytes;
+    if (g > band_row.back() && rows_bytes > band_cap) band_row.push_back(g);
   }
-  band_row.push_back(NB);
+  band_row.push_back(NG);
   std::vector<int2> launch;
   launch.reserve(tab.size());
   int max_band = 1;
@@ -507,7 +512,8 @@ static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap) 
   const size_t nw = std::max<size_t>(launch.size(), 1), nb_w = (size_t)max_band;
   int rc = OD_OK;
   if ((rc = dalloc(&c->wg_tab, nw * sizeof(int2))) || (rc = dalloc(&c->wg_base, base.size() * sizeof(int))) ||
-      (rc = dalloc(&c->sym_tp, nb_w * 3 * SYM_TB * 8)) || (rc = dalloc(&c->sym_sp, nb_w * 3 * ch * SYM_TB * 4)))
+      (rc = dalloc(&c->sym_tp, nb_w * R * 3 * SYM_TB * 8)) ||
+      (rc = dalloc(&c->sym_sp, nb_w * 3 * ch * SYM_TB * 4)))
     return rc;
   c->band_row = band_row;
   c->band_slot.assign(band_row.size(), 0);
@@ -516,6 +522,7 @@ static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap) 
   HIPCHK(hipMemcpy(c->wg_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
   c->n_blocks = NB;
   c->sym_ch = ch;
+  c->sym_R = R;
   c->n_wg = (int)n_wg;
   return OD_OK;
 }
@@ -597,10 +604,11 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   const int ch = sym_default_chunk(be, cfg->sym_chunk);
   c->band_cap = cfg->sym_band_mib > 0 ? cfg->sym_band_mib * 1048576.0 : SYM_BAND_BYTES;
   int rc = OD_OK;
-  if ((rc = sym_build(c, be, ch, c->band_cap)) || (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)) ||
-      (rc = dalloc(&c->sym_acc, (size_t)3 * c->n_pad * 8)))
+  const int var = sym_default_variant(sym_wg_count(be, ch), cfg->sym_tile);
+  if ((rc = sym_build(c, be, ch, c->band_cap, sym_variants[var].rows)) ||
+      (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)) || (rc = dalloc(&c->sym_acc, (size_t)3 * c->n_pad * 8)))
     return rc;
-  c->sym_var = sym_default_variant(c->n_wg, cfg->sym_tile);
+  c->sym_var = var;
   c->sym = 1;
   return OD_OK;
 }
@@ -723,31 +731,31 @@ static int launch_pair(od_ctx* c, const void* pi, const void* pj, double* F, boo
     const SymVariant& V = sym_variants[c->sym_var];
     const int nbands = (int)c->band_row.size() - 1;
     if (nbands == 1) {
-      hipLaunchKernelGGL(V.fn, dim3(c->n_wg), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
-                         c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, 0, a, linv, (const int*)nullptr,
-                         (const float*)nullptr, c->sym_tp, c->sym_sp);
+      hipLaunchKernelGGL(V.fn, dim3(c->n_wg), dim3(V.rows * SYM_TB / V.mt), 0, c->stream, (const float4*)pj,
+                         c->n_blocks, c->sym_ch, (const int2*)c->wg_tab, (const int*)c->wg_base, 0, a, linv,
+                         (const int*)nullptr, (const float*)nullptr, c->sym_tp, c->sym_sp);
       if (timed) HIPCHK(hipEventRecord(c->ev1, c->stream));
       hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
                          (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
       hipLaunchKernelGGL(k_sym_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->sym_dg, c->sym_tp,
-                         c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->n_pad, -C * L * L, (const int*)nullptr,
-                         (const double*)nullptr, F);
+                         c->sym_sp, c->wg_base, c->n_blocks, c->sym_ch, c->sym_R, c->n_pad, -C * L * L,
+                         (const int*)nullptr, (const double*)nullptr, F);
       return OD_OK;
     }
-    // bands of rows, each launched and reduced into acc in order (the timed
-    // region then holds the band reductions between the launches)
+    // bands of row groups, each launched and reduced into acc in order (the
+    // timed region then holds the band reductions between the launches)
     hipLaunchKernelGGL(k_pair32_diag, dim3(c->n_blocks * (SYM_TB / 256)), dim3(256), 0, c->stream,
                        (const float4*)pj, c->n_pad, a, linv, (const float*)nullptr, c->sym_dg);
     for (int b = 0; b < nbands; ++b) {
       const int w0 = c->band_slot[b], nw = c->band_slot[b + 1] - w0;
       if (nw > 0)
-        hipLaunchKernelGGL(V.fn, dim3(nw), dim3(SYM_TB / V.mt), 0, c->stream, (const float4*)pj, c->n_blocks,
-                           c->sym_ch, (const int2*)(c->wg_tab + w0), (const int*)c->wg_base, w0, a, linv,
-                           (const int*)nullptr, (const float*)nullptr, c->sym_tp, c->sym_sp);
+        hipLaunchKernelGGL(V.fn, dim3(nw), dim3(V.rows * SYM_TB / V.mt), 0, c->stream, (const float4*)pj,
+                           c->n_blocks, c->sym_ch, (const int2*)(c->wg_tab + w0), (const int*)c->wg_base, w0, a,
+                           linv, (const int*)nullptr, (const float*)nullptr, c->sym_tp, c->sym_sp);
       if (timed && b == nbands - 1) HIPCHK(hipEventRecord(c->ev1, c->stream));
       hipLaunchKernelGGL(k_sym_band_reduce, dim3(c->n_pad / 256), dim3(256), 0, c->stream, (const double*)c->sym_dg,
                          (const double*)c->sym_tp, (const float*)c->sym_sp, (const int*)c->wg_base, c->band_row[b],
-                         c->band_row[b + 1], w0, c->sym_ch, c->n_pad, -C * L * L, b == 0 ? 1 : 0,
+                         c->band_row[b + 1], w0, c->sym_ch, c->sym_R, c->n_pad, -C * L * L, b == 0 ? 1 : 0,
                          b == nbands - 1 ? 1 : 0, c->sym_acc, F);
     }
     return OD_OK;
@@ -1462,8 +1470,11 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
   c->graph = nullptr;
   c->graph_steps = 0;
   auto time_one = [&](int ch, int var) -> int {
-    if (sym_wg_count(be, ch) == 0 || sym_bytes(be, ch) > skip_cap) return OD_OK;
-    if (c->sym_ch != ch) CHKRET(sym_build(c, be, ch, band_cap));
+    const int R = sym_variants[var].rows;
+    // row groups need one simulation (a group must not straddle two)
+    if (R > 1 && std::is_same<C, od_batch>::value) return OD_OK;
+    if (sym_wg_count(be, ch, R) == 0 || sym_bytes(be, ch, R) > skip_cap) return OD_OK;
+    if (c->sym_ch != ch || c->sym_R != R) CHKRET(sym_build(c, be, ch, band_cap, R));
     c->sym_var = var;
     evaluate();  // warm-up
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1489,7 +1500,8 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
     for (int ch : kTuneChunks)
       if (ch != ch1 && (ch == 2 * ch1 || 2 * ch == ch1)) CHKRET(time_one(ch, best_var));
   }
-  if (c->sym_ch != best_ch) CHKRET(sym_build(c, be, best_ch, band_cap));
+  if (c->sym_ch != best_ch || c->sym_R != sym_variants[best_var].rows)
+    CHKRET(sym_build(c, be, best_ch, band_cap, sym_variants[best_var].rows));
   c->sym_var = best_var;
   *chosen_ch = best_ch;
   *chosen_var = best_var;
@@ -1499,11 +1511,12 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
 extern "C" int od_sym_variants(void) { return SYM_N_VARIANTS; }
 
 extern "C" int od_sym_variant_info(int32_t v, int32_t* targets_per_lane, int32_t* waves_per_simd,
-                                   int32_t* vgpr_operands) {
+                                   int32_t* vgpr_operands, int32_t* target_blocks) {
   if (v < 1 || v > SYM_N_VARIANTS) return set_err(OD_ERR_ARG, "variant %d outside [1, %d]", v, SYM_N_VARIANTS);
   if (targets_per_lane) *targets_per_lane = sym_variants[v - 1].mt;
   if (waves_per_simd) *waves_per_simd = sym_variants[v - 1].wpe;
   if (vgpr_operands) *vgpr_operands = sym_variants[v - 1].vgpr_step;
+  if (target_blocks) *target_blocks = sym_variants[v - 1].rows;
   return OD_OK;
 }
 

@@ -198,7 +198,8 @@ def load_library(path=LIB_PATH):
     lib.od_tune.argtypes = [c_ctx, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32)]
     lib.od_sym_variants.argtypes = []
     lib.od_sym_choice.argtypes = [c_ctx, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]
-    lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]
+    lib.od_sym_variant_info.argtypes = [ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32),
+                                        P(ctypes.c_int32)]
     lib.od_update_config.argtypes = [c_ctx, P(Config)]
     lib.od_write_xyz.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_int64]
     lib.od_write_csv.argtypes = [ctypes.c_char_p, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
@@ -239,15 +240,16 @@ def check_abi():
 
 
 def sym_variants():
-    """[(64-target sub-blocks per wave, waves per SIMD, VGPR operands 0/1)] of
-    the symmetric kernel variants; od_config.sym_tile = index + 1 selects one."""
+    """[(64-target sub-blocks per wave, waves per SIMD, VGPR operands 0/1,
+    target blocks per work-group)] of the symmetric kernel variants;
+    od_config.sym_tile = index + 1 selects one."""
     lib = load_library()
     out = []
     for v in range(1, lib.od_sym_variants() + 1):
-        mt, wps, vg = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(lib.od_sym_variant_info(v, ctypes.byref(mt), ctypes.byref(wps), ctypes.byref(vg)),
+        mt, wps, vg, rows = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.od_sym_variant_info(v, ctypes.byref(mt), ctypes.byref(wps), ctypes.byref(vg), ctypes.byref(rows)),
                "od_sym_variant_info")
-        out.append((mt.value, wps.value, vg.value))
+        out.append((mt.value, wps.value, vg.value, rows.value))
     return out
 
 

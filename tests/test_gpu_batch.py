@@ -185,11 +185,19 @@ def test_batch_tune_is_reproducible():
             say("  chosen sym_chunk, sym_tile", chosen)
             assert chosen[0] in (1, 2, 4, 8, 16, 32)
             assert 1 <= chosen[1] <= len(engine.sym_variants())
+            # row groups need one simulation: the batch tuner never picks them
+            assert engine.sym_variants()[chosen[1] - 1][3] == 1
         rec = b.run(12, engine.RUN_GRAPH)
         outs.append((rec, [b.get_ions(s) for s in range(len(cfgs))]))
         b.close()
     cfgs[0].sym_chunk = cfgs[0].sym_tile = 0
     assert np.array_equal(outs[0][0], outs[1][0])
+    grouped = [i + 1 for i, v in enumerate(engine.sym_variants()) if v[3] > 1]
+    assert grouped
+    cfgs[0].sym_tile = grouped[0]
+    with pytest.raises(RuntimeError, match="groups target blocks"):
+        engine.Batch(cfgs)
+    cfgs[0].sym_tile = 0
     for (x, v), (x1, v1) in zip(outs[0][1], outs[1][1]):
         assert np.array_equal(x, x1) and np.array_equal(v, v1)
 

@@ -97,7 +97,7 @@ def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
     """Every variant of the symmetric kernel the autotuner may pick (tile
     shape, waves per SIMD) meets the FP32c force bound against the oracle, for
@@ -127,10 +127,13 @@ def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
         eng.Engine(bad)
 
 
-def test_sym_schedule_in_bands(oracle_lib):
+@pytest.mark.parametrize("tile", [0, 10])
+def test_sym_schedule_in_bands(oracle_lib, tile):
     """The symmetric kernel's triangular schedule run in bands of rows (each
     launched and reduced in order into an FP64 accumulator; the default band
-    is 48 GB of partials, here 1 MiB, so every band holds one or two rows):
+    is 48 GB of partials, here 1 MiB, so every band holds one or two rows),
+    for the default variant and for row groups of 4 target blocks (variant
+    10: one source partial per work-group for all 4 blocks' waves):
     forces within the FP32c bound of the oracle and within 1e-6 of the
     one-band schedule (only the summation order differs; the engine reports
     one band, then several), and a run is bitwise reproducible, eager and
@@ -144,6 +147,7 @@ def test_sym_schedule_in_bands(oracle_lib):
     (FP64 on rounded positions vs exact, + 5e-6)."""
     eng = _engine_mod()
     od, cfg, x0, v0 = _setup(N=20000, precision="fp32c", craft=False, mode="sym")
+    cfg.sym_tile = tile
     f_orc = oracle_lib.pair_forces(cfg, x0)
     f_rnd = oracle_lib.pair_forces(cfg, oracle_lib.fp32c_rounded_positions(cfg, x0))
     fn = np.linalg.norm(f_rnd, axis=1)
@@ -176,6 +180,7 @@ def test_sym_schedule_in_bands(oracle_lib):
     # steps with ions crafted into the dust and beyond the walls
     od, cfg, x0, v0 = _setup(N=20000, precision="fp32c", craft=True, mode="sym")
     cfg.sym_band_mib = 1
+    cfg.sym_tile = tile
     outs = []
     for flags in (0, 0, eng.RUN_GRAPH):
         e = eng.Engine(cfg)
@@ -190,8 +195,9 @@ def test_sym_schedule_in_bands(oracle_lib):
 def test_full_size_sym_forces_vs_fp64_gpu():
     """At the benchmark size, 2^20 ions, the FP32c symmetric kernel against the
     plain FP64 row kernel on the GPU (independent code: ordered pairs, FP64
-    law, no partials), for the default variant and schedule and for 4 targets
-    per lane with 16 source blocks per work-group.
+    law, no partials), for the default variant and schedule, for 4 targets
+    per lane with 16 source blocks per work-group, and for row groups of 4
+    target blocks (variant 10) with 8.
 
     Two error sources are separated.  (a) Kernel arithmetic: against FP64 on
     the same positions rounded to FP32 in the kernel's units (what the FP32
@@ -219,7 +225,7 @@ def test_full_size_sym_forces_vs_fp64_gpu():
     f64r = e.pair_forces(x32)
     e.close()
     cfg32, _, _ = workloads.prepare(od, alpha=alpha, np_seed=5, precision="fp32c")
-    for tile, ch in ((0, 0), (3, 16)):
+    for tile, ch in ((0, 0), (3, 16), (10, 8)):
         cfg32.sym_tile, cfg32.sym_chunk = tile, ch
         e = eng.Engine(cfg32)
         assert e.pair_info()[0] == MODES["sym"]
