@@ -41,6 +41,8 @@ SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes pe
     (4, 1): 31.020e9, (4, 2): 22.758e9, (4, 4): 18.547e9, (4, 8): 16.588e9, (4, 16): 15.480e9, (4, 32): 14.710e9,
     (6, 1): 27.432e9, (6, 2): 20.433e9, (6, 4): 16.871e9, (6, 8): 14.969e9, (6, 16): 13.856e9, (6, 32): 13.145e9,
     (7, 1): 27.440e9, (7, 2): 20.433e9, (7, 4): 16.845e9, (7, 8): 14.889e9, (7, 16): 13.907e9, (7, 32): 13.185e9,
+    (9, 4): 10.382e9, (9, 8): 8.599e9, (9, 16): 7.624e9,
+    (10, 4): 7.406e9, (10, 8): 5.461e9, (10, 16): 4.453e9, (10, 32): 3.894e9,
 }
 # variants whose registers spill (scratch traffic of their own); for the others
 # the schedule fixes the traffic (partials and positions): variants 1 and 3
@@ -552,8 +554,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            # profiled per (variant, schedule); the untuned default at 2^20 is variant 6 with 4 source
-            # blocks per work-group
+            # profiled per (variant, schedule); the untuned default at 2^20 is variant 10 (row groups of
+            # 4 target blocks) with 8 source blocks per work-group
             "traffic": traffic,
             "traffic_source": traffic_src,
             "traffic_unit": "bytes per launch",

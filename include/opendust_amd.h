@@ -87,8 +87,10 @@ typedef struct od_config {
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
   int32_t sym_tile;      /* symmetric kernel variant (tile shape, waves per SIMD, operand form),
-                            1..od_sym_variants(); 0 = automatic (variant 6 with at least 2048
-                            work-groups, else 3).  Cutoff mode, FP32c: the symmetric cell kernel's
+                            1..od_sym_variants(); 0 = automatic (one simulation: variant 10, row
+                            groups of 4 target blocks, with 8 source blocks per work-group when
+                            that gives at least 4096 work-groups; else variant 6 with at least
+                            2048 work-groups, else 3).  Cutoff mode, FP32c: the symmetric cell kernel's
                             64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4; 0 = 1 */
   int32_t sym_band_mib;  /* symmetric kernel: partials per band of rows of its triangular schedule, MiB;
                             0 = 48 GB (larger schedules, about 2^21 ions and up, run in several bands) */
@@ -245,9 +247,12 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* b
  * pair evaluation `reps` times per candidate on the current ions: first 1, 2,
  * 4, 8, 16 and 32 source blocks per work-group with the configured kernel
  * variant, then every kernel variant (tile shape: 4, 8 or 16 64-target
- * sub-blocks per wave; register budget: waves per SIMD) at the best of those,
- * then the neighbouring block counts with the best variant.  It keeps the
- * fastest and recomputes the ion-ion force of the current positions with it.
+ * sub-blocks per wave; register budget: waves per SIMD; operand form; target
+ * blocks per work-group, one simulation only) at the best of those, then the
+ * neighbouring block counts with the best variant.  Among the candidates
+ * within 1 % of the fastest (ties at the timing's noise) it keeps the one
+ * whose schedule writes the fewest partial bytes, and recomputes the ion-ion
+ * force of the current positions with it.
  * Both choices group the partial sums, so results are reproducible for a
  * given pair: *sym_chunk and *sym_tile return it, and od_config.sym_chunk and
  * od_config.sym_tile fix it.  Call after od_set_ions.  No counterpart in the

@@ -527,7 +527,9 @@ static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap, 
   return OD_OK;
 }
 
-// default kernel variant: the configured one, else 8 target sub-blocks per
+// default kernel variant with one target block per work-group (batches, and
+// single simulations too small for row groups, see setup_sym): the
+// configured one, else 8 target sub-blocks per
 // wave (2 waves per work-group) with VGPR operands when the schedule has at
 // least 2048 work-groups (4 waves per SIMD), else 4 (4 waves per work-group:
 // at 2^15 ions the 496 work-groups then still give about 2 waves per SIMD).
@@ -601,10 +603,22 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   // 2^15 0.47 ms symmetric vs 0.62 ms rows, 2^14 0.30 vs 0.22 ms)
   if (T < 2 || (!forced && c->n_pad < (1 << 15))) return OD_OK;
   const std::vector<int> be(T, T);
-  const int ch = sym_default_chunk(be, cfg->sym_chunk);
+  int ch = sym_default_chunk(be, cfg->sym_chunk);
   c->band_cap = cfg->sym_band_mib > 0 ? cfg->sym_band_mib * 1048576.0 : SYM_BAND_BYTES;
   int rc = OD_OK;
-  const int var = sym_default_variant(sym_wg_count(be, ch), cfg->sym_tile);
+  int var = sym_default_variant(sym_wg_count(be, ch), cfg->sym_tile);
+  // default for large N: row groups of 4 target blocks with 8 source blocks
+  // per work-group (variant 10), when that still gives >= 4096 work-groups of
+  // 8 waves.  Measured at 2^20: 288.6 ms per launch and 292.3 ms per step
+  // against 288.4 and 293.5 with variant 6 at 4 blocks, and 5.5 GB of HBM
+  // traffic per launch against 16.9 GB.
+  if (cfg->sym_tile == 0) {
+    const int ch4 = cfg->sym_chunk > 0 ? cfg->sym_chunk : 8;
+    if (sym_wg_count(be, ch4, 4) >= 4096) {
+      var = SYM_GROUPED_DEFAULT;
+      ch = ch4;
+    }
+  }
   if ((rc = sym_build(c, be, ch, c->band_cap, sym_variants[var].rows)) ||
       (rc = dalloc(&c->sym_dg, (size_t)3 * c->n_pad * 8)) || (rc = dalloc(&c->sym_acc, (size_t)3 * c->n_pad * 8)))
     return rc;
@@ -1462,6 +1476,11 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
                     double band_cap, double skip_cap) {
   double best_ms = 1e300;
   int best_ch = c->sym_ch, best_var = c->sym_var;
+  struct Cand {
+    double ms, bytes;
+    int ch, var;
+  };
+  std::vector<Cand> cands;
   // a captured step holds the schedule's buffers, which sym_build replaces:
   // drop it before the first rebuild, so no error path can leave it behind
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1484,6 +1503,7 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
     HIPCHK(hipGetLastError());
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    cands.push_back({(double)ms, sym_bytes(be, ch, R), ch, var});
     if (ms < best_ms) {
       best_ms = ms;
       best_ch = ch;
@@ -1500,6 +1520,17 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
     for (int ch : kTuneChunks)
       if (ch != ch1 && (ch == 2 * ch1 || 2 * ch == ch1)) CHKRET(time_one(ch, best_var));
   }
+  // candidates within 1 % of the fastest are ties at this timing's noise:
+  // among them, keep the one whose schedule writes the fewest partial bytes
+  // (HBM traffic; faster first among equal bytes)
+  double sel_bytes = 1e300, sel_ms = 1e300;
+  for (const Cand& k : cands)
+    if (k.ms <= best_ms * 1.01 && (k.bytes < sel_bytes || (k.bytes == sel_bytes && k.ms < sel_ms))) {
+      sel_bytes = k.bytes;
+      sel_ms = k.ms;
+      best_ch = k.ch;
+      best_var = k.var;
+    }
   if (c->sym_ch != best_ch || c->sym_R != sym_variants[best_var].rows)
     CHKRET(sym_build(c, be, best_ch, band_cap, sym_variants[best_var].rows));
   c->sym_var = best_var;

@@ -317,7 +317,8 @@ def test_bench_traffic_lookup_labels_its_source():
     t, src = bench.sym_traffic_2e20(8, 8)
     assert t == bench.SYM_TRAFFIC_2E20[(1, 8)] and "variant 1" in src
     assert bench.sym_traffic_2e20(4, 64) == (None, None)
-    assert bench.sym_traffic_2e20(10, 8, rows=4) == (None, None)
+    assert (10, 1) not in bench.SYM_TRAFFIC_2E20 and (1, 1) in bench.SYM_TRAFFIC_2E20
+    assert bench.sym_traffic_2e20(10, 1, rows=4) == (None, None)  # grouped: no fallback to variant 1
     assert all(v not in bench.SYM_SPILLING_VARIANTS for v in (1, 3, 5, 6, 7, 8))
 
 
