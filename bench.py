@@ -32,21 +32,23 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector with packed ops; also matrix), 
 FLOP_PER_PAIR = 24         # algorithmic FLOPs of one pair term
 # HBM bytes per k_pair32_sym launch at 2^20 ions for a given kernel variant and
 # number of source blocks per work-group, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# scaled by a 1 GiB copy's readings (x2.000, x1.000): profiles/k_pair32_sym_2e20_pmc.md
+# scaled by a 1 GiB copy's readings (x2.000, x1.000), with the XCD-aware tile launch
+# order of the engine: profiles/k_pair32_sym_2e20_pmc.md
 # (variants 2 and 4 spill registers; their scratch traffic is included)
 SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes per launch
-    (1, 1): 27.430e9, (1, 2): 20.442e9, (1, 4): 16.905e9, (1, 8): 15.002e9, (1, 16): 13.873e9, (1, 32): 13.267e9,
-    (2, 1): 43.531e9, (2, 2): 30.513e9, (2, 4): 23.763e9, (2, 8): 20.361e9, (2, 16): 18.394e9, (2, 32): 17.445e9,
-    (3, 1): 27.449e9, (3, 2): 20.549e9, (3, 4): 17.092e9, (3, 8): 15.010e9, (3, 16): 13.793e9, (3, 32): 12.922e9,
-    (4, 1): 31.020e9, (4, 2): 22.758e9, (4, 4): 18.547e9, (4, 8): 16.588e9, (4, 16): 15.480e9, (4, 32): 14.710e9,
-    (6, 1): 27.432e9, (6, 2): 20.433e9, (6, 4): 16.871e9, (6, 8): 14.969e9, (6, 16): 13.856e9, (6, 32): 13.145e9,
-    (7, 1): 27.440e9, (7, 2): 20.433e9, (7, 4): 16.845e9, (7, 8): 14.889e9, (7, 16): 13.907e9, (7, 32): 13.185e9,
-    (9, 4): 10.382e9, (9, 8): 8.599e9, (9, 16): 7.624e9,
-    (10, 4): 7.406e9, (10, 8): 5.461e9, (10, 16): 4.453e9, (10, 32): 3.894e9,
+    (1, 1): 20.625e9, (1, 2): 13.926e9, (1, 4): 10.656e9, (1, 8): 9.014e9, (1, 16): 8.275e9, (1, 32): 7.890e9,
+    (2, 1): 40.530e9, (2, 2): 25.591e9, (2, 4): 18.874e9, (2, 8): 14.837e9, (2, 16): 13.301e9, (2, 32): 12.711e9,
+    (3, 1): 19.686e9, (3, 2): 13.698e9, (3, 4): 10.542e9, (3, 8): 9.070e9, (3, 16): 8.286e9, (3, 32): 7.694e9,
+    (4, 1): 26.310e9, (4, 2): 16.920e9, (4, 4): 12.250e9, (4, 8): 10.366e9, (4, 16): 9.452e9, (4, 32): 9.830e9,
+    (6, 1): 20.625e9, (6, 2): 14.032e9, (6, 4): 10.691e9, (6, 8): 9.057e9, (6, 16): 8.325e9, (6, 32): 8.095e9,
+    (7, 1): 20.721e9, (7, 2): 14.036e9, (7, 4): 10.715e9, (7, 8): 9.090e9, (7, 16): 8.332e9, (7, 32): 8.110e9,
+    (9, 4): 7.225e9, (9, 8): 5.726e9, (9, 16): 4.930e9,
+    (10, 4): 6.524e9, (10, 8): 4.319e9, (10, 16): 3.241e9, (10, 32): 2.803e9,
 }
 # variants whose registers spill (scratch traffic of their own); for the others
-# the schedule fixes the traffic (partials and positions): variants 1 and 3
-# agree within 2.6 % at every profiled block count, 1, 6 and 7 within 1 %
+# the schedule fixes the traffic: the partial-sum writes are identical and the
+# position reads (0.4-1.5 GB) differ, so variants 1, 3, 6 and 7 agree within
+# 5 % at every profiled block count
 SYM_SPILLING_VARIANTS = (2, 4)
 
 
