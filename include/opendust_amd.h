@@ -87,11 +87,13 @@ typedef struct od_config {
   int32_t sym_chunk;     /* symmetric kernel: 1024-ion source blocks per workgroup; 0 = automatic */
   int32_t sim_id;        /* Philox counter word 2 (index of the simulation in a sweep; 0 alone) */
   int32_t sym_tile;      /* symmetric kernel variant (tile shape, waves per SIMD, operand form),
-                            1..od_sym_variants(); 0 = automatic (one simulation: variant 10, row
-                            groups of 4 target blocks, with 8 source blocks per work-group when
-                            that gives at least 4096 work-groups; else variant 6 with at least
-                            2048 work-groups, else 3).  Cutoff mode, FP32c: the symmetric cell kernel's
-                            64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4; 0 = 1 */
+                            1..od_sym_variants(); 0 = automatic (one simulation: variant 12, row
+                            groups of 4 target blocks, sources read from LDS, with 8 source blocks
+                            per work-group when that gives at least 4096 work-groups; else variant
+                            11 with at least 2048 work-groups, else 3).  Cutoff mode, FP32c: the
+                            symmetric cell kernel's 64-target sub-blocks per wave, 1, 2, 3 = 1, 2, 4
+                            with sources rotated through registers, 4, 5, 6 = 1, 2, 4 with sources
+                            read from LDS; 0 = 5 */
   int32_t sym_band_mib;  /* symmetric kernel: partials per band of rows of its triangular schedule, MiB;
                             0 = 48 GB (larger schedules, about 2^21 ions and up, run in several bands) */
   uint64_t seed;         /* Philox4x32-10 key */
@@ -242,7 +244,7 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* b
 
 /* Autotuner of the symmetric FP32 pair kernel (contexts that run it; others
  * return *sym_chunk = *sym_tile = 0 and change nothing; cutoff mode, FP32c:
- * it times the symmetric cell kernel's three variants on the current ions, keeps
+ * it times the symmetric cell kernel's six variants on the current ions, keeps
  * the fastest and returns *sym_chunk = 0, *sym_tile = the variant).  It times the whole
  * pair evaluation `reps` times per candidate on the current ions: first 1, 2,
  * 4, 8, 16 and 32 source blocks per work-group with the configured kernel
@@ -260,8 +262,11 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* b
 int od_tune(od_ctx* ctx, int32_t reps, int32_t* sym_chunk, int32_t* sym_tile);
 
 /* Number of symmetric kernel variants, and the tile shape (64-target
- * sub-blocks per wave), waves per SIMD, operand form (1: the pair step's
- * full-rate instructions take VGPR operands only, 0: DPP/SGPR operands) and
+ * sub-blocks per wave), waves per SIMD, operand form (0: sources rotated
+ * through DPP operands of the subtractions, SGPR constants; 1: sources and
+ * reactions rotated by DPP moves, every full-rate instruction with VGPR
+ * operands; 2: sources read from the wave's LDS copy, reactions rotated by
+ * DPP moves, VGPR operands) and
  * target blocks per work-group (> 1: row groups that share each source
  * partial, one simulation only, not in batches) of variant v (1-based) as
  * compiled.  Measurement support. */

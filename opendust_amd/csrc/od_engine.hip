@@ -590,16 +590,17 @@ static int sym_build(C* c, const std::vector<int>& be, int ch, double band_cap, 
 
 // default kernel variant with one target block per work-group (batches, and
 // single simulations too small for row groups, see setup_sym): the
-// configured one, else 8 target sub-blocks per
-// wave (2 waves per work-group) with VGPR operands when the schedule has at
-// least 2048 work-groups (4 waves per SIMD), else 4 (4 waves per work-group:
-// at 2^15 ions the 496 work-groups then still give about 2 waves per SIMD).
+// configured one, else 8 target sub-blocks per wave (2 waves per work-group)
+// with the sources read from LDS (variant 11) when the schedule has at least
+// 2048 work-groups (4 waves per SIMD), else 4 (4 waves per work-group: at
+// 2^15 ions the 496 work-groups then still give about 2 waves per SIMD).
 // Measured on MI355X at 2^15 ions: 0.33 ms per evaluation with 4, 0.64 ms with
 // 8; at 2^20, 4 source blocks per work-group: 308.5 ms with the DPP/SGPR
-// operand step, 287.7 ms with the VGPR operand step.
+// operand step, 287.7 ms with the VGPR operand step, 278.8 ms with the
+// sources from LDS.
 static int sym_default_variant(int64_t n_wg, int configured) {
   if (configured > 0) return configured - 1;
-  return n_wg >= 2048 ? 5 : 2;
+  return n_wg >= 2048 ? 10 : 2;
 }
 
 // default source blocks per work-group: the largest of 4, 2, 1 that still
@@ -669,10 +670,11 @@ static int setup_sym(od_ctx* c, const od_config* cfg) {
   int rc = OD_OK;
   int var = sym_default_variant(sym_wg_count(be, ch), cfg->sym_tile);
   // default for large N: row groups of 4 target blocks with 8 source blocks
-  // per work-group (variant 10), when that still gives >= 4096 work-groups of
-  // 8 waves.  Measured at 2^20: 288.6 ms per launch and 292.3 ms per step
-  // against 288.4 and 293.5 with variant 6 at 4 blocks, and 5.5 GB of HBM
-  // traffic per launch against 16.9 GB.
+  // per work-group and the sources read from LDS (variant 12), when that
+  // still gives >= 4096 work-groups of 8 waves.  Measured at 2^20: 279.3 ms
+  // per launch (288.5 with the VGPR-operand step, variant 10), 4.3 GB of HBM
+  // traffic per launch against 16.9 GB with one target block and 4 source
+  // blocks per work-group.
   if (cfg->sym_tile == 0) {
     const int ch4 = cfg->sym_chunk > 0 ? cfg->sym_chunk : 8;
     if (sym_wg_count(be, ch4, 4) >= 4096) {

@@ -267,6 +267,72 @@ __device__ __forceinline__ void sym_phase_v(SymT<MT>& T, float& xs, float& ys, f
   for (int st = 0; st < 16; ++st) sym_step_v<MT, CUT>(T, xs, ys, zs, bx, by, bz, L, eps);
 }
 
+// v rotated by one lane across the whole wave (DPP wave_ror:1): lane i
+// receives lane i - 1's value, lane 0 lane 63's.
+__device__ __forceinline__ float wave_ror1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                               0x13C, 0xF, 0xF, false));
+}
+
+// The 64 x 64 pairs of a wave's source sub-block with the sources read from
+// LDS instead of rotated through registers.  The wave's private LDS region
+// holds its 64 sources twice (ls[j] = ls[j + 64] = source j); at step s lane
+// l reads source (l - s) mod 64 at ls[l + 64 - s] (the step in the immediate
+// offset), and the reaction sums travel with the sources: one wave_ror:1 per
+// component and step, so after 64 steps lane l holds the reaction on its own
+// source.  Per step: MT pair terms, 3 DPP moves, one LDS read.
+// ls: this lane's entry ls + lane + 64.  (bx, by, bz): added to.
+template <int MT, int CUT>
+__device__ __forceinline__ void sym_tile_lds(SymT<MT>& T, const float4* ls, float& bx, float& by, float& bz,
+                                             const SymLaw& L) {
+  const float eps = vgpr(EPS32);
+  for (int s0 = 0; s0 < 64; s0 += 16) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const float4 q = ls[-(s0 + s)];
+#pragma unroll
+      for (int k = 0; k < MT; ++k) {
+        const float dx = q.x - T.x[k];
+        const float dy = q.y - T.y[k];
+        const float dz = q.z - T.z[k];
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps)));
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const float e = __builtin_amdgcn_exp2f(-r);
+        const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(L.a, r, r2));
+        const float t = __builtin_fmaf(r, w, L.linv);
+        float h = e * w * t;
+        if constexpr (CUT == 2) {
+          const float x = __builtin_fminf(__builtin_fmaxf((r - L.rs) * L.invw, 0.0f), 1.0f);
+          const float x2 = x * x;
+          const float Sx = __builtin_fmaf(x2 * x, __builtin_fmaf(x, __builtin_fmaf(x, -6.0f, 15.0f), -10.0f), 1.0f);
+          const float omx = 1.0f - x;
+          const float dS = -30.0f * x2 * omx * omx * L.invw;
+          h = __builtin_fmaf(h, Sx, -(e * w * dS));
+        }
+        if constexpr (CUT >= 1) h = (r < L.rc) ? h : 0.0f;
+        T.ax[k] = __builtin_fmaf(h, dx, T.ax[k]);
+        T.ay[k] = __builtin_fmaf(h, dy, T.ay[k]);
+        T.az[k] = __builtin_fmaf(h, dz, T.az[k]);
+        bx = __builtin_fmaf(-h, dx, bx);
+        by = __builtin_fmaf(-h, dy, by);
+        bz = __builtin_fmaf(-h, dz, bz);
+      }
+      bx = wave_ror1(bx);
+      by = wave_ror1(by);
+      bz = wave_ror1(bz);
+      pin(bx);
+      pin(by);
+      pin(bz);
+#pragma unroll
+      for (int k = 0; k < MT; ++k) {
+        pin(T.ax[k]);
+        pin(T.ay[k]);
+        pin(T.az[k]);
+      }
+    }
+  }
+}
+
 // The 64 x 64 pairs of the wave's source sub-block (xj, yj, zj, one per lane)
 // with its MT target sub-blocks: four phases of 16 rotation steps, rows
 // exchanging sources between phases.  On return (bx, by, bz) hold the

@@ -72,7 +72,7 @@ def test_invalid_config_is_an_error_not_a_crash(eng):
 def test_sym_variant_table_and_range(eng):
     v = eng.sym_variants()
     assert v[0] == (8, 3, 0, 1) and (8, 3, 1, 1) in v and (8, 4, 1, 4) in v and len(v) >= 2
-    assert all(mt in (4, 8, 16) and 1 <= w <= 8 and vg in (0, 1) and rows in (1, 2, 4) for mt, w, vg, rows in v)
+    assert all(mt in (4, 8, 16) and 1 <= w <= 8 and vg in (0, 1, 2) and rows in (1, 2, 4) for mt, w, vg, rows in v)
     lib = eng.load_library()
     mt = ctypes.c_int32()
     assert lib.od_sym_variant_info(len(v) + 1, ctypes.byref(mt), None, None, None) == 1

@@ -97,7 +97,7 @@ def test_pair_forces_vs_oracle(oracle_lib, precision, mode):
         assert np.all(err <= 1e-4 * fn + 1e-5 * rms)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
     """Every variant of the symmetric kernel the autotuner may pick (tile
     shape, waves per SIMD) meets the FP32c force bound against the oracle, for
@@ -127,13 +127,14 @@ def test_sym_variants_pair_forces_vs_oracle(oracle_lib, tile):
         eng.Engine(bad)
 
 
-@pytest.mark.parametrize("tile", [0, 10])
+@pytest.mark.parametrize("tile", [0, 10, 12])
 def test_sym_schedule_in_bands(oracle_lib, tile):
     """The symmetric kernel's triangular schedule run in bands of rows (each
     launched and reduced in order into an FP64 accumulator; the default band
     is 48 GB of partials, here 1 MiB, so every band holds one or two rows),
-    for the default variant and for row groups of 4 target blocks (variant
-    10: one source partial per work-group for all 4 blocks' waves):
+    for the default variant and for row groups of 4 target blocks (variants
+    10 and 12, register and LDS sources: one source partial per work-group
+    for all 4 blocks' waves):
     forces within the FP32c bound of the oracle and within 1e-6 of the
     one-band schedule (only the summation order differs; the engine reports
     one band, then several), and a run is bitwise reproducible, eager and
@@ -226,7 +227,7 @@ def test_full_size_sym_forces_vs_fp64_gpu():
     f64r = e.pair_forces(x32)
     e.close()
     cfg32, _, _ = workloads.prepare(od, alpha=alpha, np_seed=5, precision="fp32c")
-    for tile, ch in ((0, 0), (3, 16), (6, 4)):
+    for tile, ch in ((0, 0), (3, 16), (6, 4), (10, 8)):
         cfg32.sym_tile, cfg32.sym_chunk = tile, ch
         e = eng.Engine(cfg32)
         assert e.pair_info()[0] == MODES["sym"]

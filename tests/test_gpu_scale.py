@@ -120,7 +120,8 @@ def test_cutoff_dense_cells_vs_oracle(oracle_lib, precision, rc, rs):
 @pytest.mark.parametrize("rc,rs", [(2.0, None), (3.2, 3.0)])
 def test_cutoff_cell_variants_vs_oracle(oracle_lib, rc, rs):
     """Every variant of the symmetric cell kernel the cutoff autotuner may
-    pick (1, 2 or 4 target sub-blocks per wave), hard and switched law, at
+    pick (1, 2 or 4 target sub-blocks per wave, sources rotated through
+    registers or read from LDS), hard and switched law, at
     the dense workload's density: FP32c bound against the oracle on the
     rounded positions, each variant bitwise reproducible."""
     eng = _eng()
@@ -133,7 +134,7 @@ def test_cutoff_cell_variants_vs_oracle(oracle_lib, rc, rs):
     f_cut = oracle_lib.cut_force_at_cutoff(cfg)
     flip = np.sqrt(np.sum((edge * f_cut) ** 2))
     tiles = []
-    for tile in range(1, 4):
+    for tile in range(1, 7):
         cfg.sym_tile = tile
         say("dense cutoff, cell kernel variant", tile, "rc", rc, "rs", rs)
         e = eng.Engine(cfg)
@@ -144,9 +145,9 @@ def test_cutoff_cell_variants_vs_oracle(oracle_lib, rc, rs):
         assert np.array_equal(f1, f2)
         _check_rows(f1, f_r, edge, f_cut, 1e-5, 1e-6, "variant %d vs oracle on rounded positions" % tile)
         assert np.linalg.norm(f1 - f_r) <= 2e-6 * np.linalg.norm(f_r) + flip
-    assert tiles == [1, 2, 3]
+    assert tiles == [1, 2, 3, 4, 5, 6]
     bad = eng.Config.from_buffer_copy(cfg)
-    bad.sym_tile = 4
+    bad.sym_tile = 7
     with pytest.raises(RuntimeError, match="sym_tile"):
         eng.Engine(bad)
 
@@ -337,7 +338,7 @@ def test_all_pairs_2e22_symmetric_in_bands_vs_fp64():
     e = eng.Engine(cfg32)
     sched = e.sym_schedule()
     say("  schedule (source blocks per work-group, variant, bands)", sched)
-    assert e.pair_info()[0] == 2 and sched[:2] == (8, 10) and sched[2] >= 2
+    assert e.pair_info()[0] == 2 and sched[:2] == (8, 12) and sched[2] >= 2
     f32 = e.pair_forces(x0)
     e.close()
     fn = np.linalg.norm(f64r, axis=1)
