@@ -572,7 +572,8 @@ def main():
             "sym_chunk": chunk,
             "sym_tile": tile,
             "sym_bands": bands,
-            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd", "vgpr_operands",
+            # operand_form: 0 DPP/SGPR operands, 1 VGPR operands, 2 sources read from LDS
+            "sym_tile_shape": (dict(zip(("targets_per_lane", "waves_per_simd", "operand_form",
                                          "target_blocks_per_work_group"),
                                         engine.sym_variants()[tile - 1]))
                                if tile else None),

@@ -240,8 +240,9 @@ def check_abi():
 
 
 def sym_variants():
-    """[(64-target sub-blocks per wave, waves per SIMD, VGPR operands 0/1,
-    target blocks per work-group)] of the symmetric kernel variants;
+    """[(64-target sub-blocks per wave, waves per SIMD, operand form (0 DPP/SGPR
+    operands, 1 VGPR operands, 2 sources read from LDS), target blocks per
+    work-group)] of the symmetric kernel variants;
     od_config.sym_tile = index + 1 selects one."""
     lib = load_library()
     out = []
