@@ -1271,7 +1271,9 @@ static int64_t cell_candidates(const od_ctx* c, const std::vector<int>& cs, cons
     if (a0 == a1) continue;
     const int ax = A % nx, ay = A / nx;
     if (c->prec == OD_PREC_FP32C) {
-      const int G = 64 * cell_variant_mt[c->sc_var];
+      const int MTv = cell_variant_mt[c->sc_var];
+      const bool split = cell_variant_split[c->sc_var];
+      const int G = 64 * MTv;
       for (int k = 0; k < CC_NSET; ++k) {
         const bool own = k == CC_NFWD;
         int dx = 0, dy = 0;
@@ -1290,7 +1292,24 @@ static int64_t cell_candidates(const od_ctx* c, const std::vector<int>& cs, cons
             s0 = t1;
             total += (int64_t)G * up64(t1 - t0);
           }
-          if (s1 > s0) total += 2 * (int64_t)G * up64(s1 - s0);
+          if (s1 <= s0) continue;
+          if (!split) {
+            total += 2 * (int64_t)G * up64(s1 - s0);
+            continue;
+          }
+          // each 64-source chunk meets only the sub-blocks that reach it
+          int b[4], e[4], lo = 0, hi = 0;
+          if (MTv == 2)
+            cc_sub_ranges<2>(cs.data(), key.data(), cA, cb, nbz, nz, own, t0, t1, b, e);
+          else
+            cc_sub_ranges<4>(cs.data(), key.data(), cA, cb, nbz, nz, own, t0, t1, b, e);
+          for (int64_t jb = s0; jb < s1; jb += 64) {
+            if (MTv == 2)
+              cc_chunk_run<2>(b, e, (int)jb, lo, hi);
+            else
+              cc_chunk_run<4>(b, e, (int)jb, lo, hi);
+            total += 2 * 64 * 64 * (int64_t)(hi - lo + 1);
+          }
         }
       }
     } else {
