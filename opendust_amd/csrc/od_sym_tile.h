@@ -281,15 +281,28 @@ __device__ __forceinline__ float wave_ror1(float v) {
 // offset), and the reaction sums travel with the sources: one wave_ror:1 per
 // component and step, so after 64 steps lane l holds the reaction on its own
 // source.  Per step: MT pair terms, 3 DPP moves, one LDS read.
+// PF: read step s + 1's source while step s is evaluated (3 more VGPRs and
+// one address instruction per step).  Measured on MI355X: the all-pairs
+// kernel with 8 target sub-blocks 276.2 -> 269.9 ms at 2^20; the cell
+// kernel with 2 slower, 52.2 -> 57.0 ms at 2^22.
 // ls: this lane's entry ls + lane + 64.  (bx, by, bz): added to.
-template <int MT, int CUT>
+template <int MT, int CUT, bool PF = false>
 __device__ __forceinline__ void sym_tile_lds(SymT<MT>& T, const float4* ls, float& bx, float& by, float& bz,
                                              const SymLaw& L) {
   const float eps = vgpr(EPS32);
+  // PF: the read after the last step, ls[-64], is the lane's own entry
+  // ls[lane] (in bounds, unused)
+  float4 qn = ls[0];
   for (int s0 = 0; s0 < 64; s0 += 16) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
-      const float4 q = ls[-(s0 + s)];
+      float4 q;
+      if constexpr (PF) {
+        q = qn;
+        qn = ls[-(s0 + s + 1)];
+      } else {
+        q = ls[-(s0 + s)];
+      }
 #pragma unroll
       for (int k = 0; k < MT; ++k) {
         const float dx = q.x - T.x[k];
