@@ -45,8 +45,8 @@ SYM_TRAFFIC_2E20 = {  # (kernel variant, source blocks per work-group): bytes pe
     (9, 4): 7.225e9, (9, 8): 5.726e9, (9, 16): 4.930e9,
     (10, 4): 6.524e9, (10, 8): 4.319e9, (10, 16): 3.241e9, (10, 32): 2.803e9,
     (11, 1): 21.417e9, (11, 2): 14.630e9, (11, 4): 10.970e9, (11, 8): 9.214e9, (11, 16): 8.386e9, (11, 32): 8.136e9,
-    (12, 4): 6.519e9, (12, 8): 4.318e9, (12, 16): 3.247e9, (12, 32): 2.825e9,
-    (13, 4): 7.779e9, (13, 8): 5.937e9, (13, 16): 5.052e9,
+    (12, 1): 18.450e9, (12, 2): 11.139e9, (12, 4): 6.519e9, (12, 8): 4.318e9, (12, 16): 3.247e9, (12, 32): 2.825e9,
+    (13, 1): 21.992e9, (13, 2): 11.048e9, (13, 4): 7.779e9, (13, 8): 5.937e9, (13, 16): 5.052e9, (13, 32): 4.665e9,
 }
 # variants whose registers spill (scratch traffic of their own); for the others
 # the schedule fixes the traffic: the partial-sum writes are identical and the
