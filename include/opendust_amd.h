@@ -252,7 +252,7 @@ int od_sym_choice(od_ctx* ctx, int32_t* sym_chunk, int32_t* sym_tile, int32_t* b
  * sub-blocks per wave; register budget: waves per SIMD; operand form; target
  * blocks per work-group, one simulation only) at the best of those, then the
  * neighbouring block counts with the best variant.  Among the candidates
- * within 0.5 % of the fastest (ties at the timing's noise) it keeps the one
+ * within 1 % of the fastest (counted as ties) it keeps the one
  * whose schedule writes the fewest partial bytes, and recomputes the ion-ion
  * force of the current positions with it.
  * Both choices group the partial sums, so results are reproducible for a

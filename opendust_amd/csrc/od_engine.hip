@@ -1602,13 +1602,13 @@ static int tune_sym(C* c, const std::vector<int>& be, int reps, F&& evaluate, in
     for (int ch : kTuneChunks)
       if (ch != ch1 && (ch == 2 * ch1 || 2 * ch == ch1)) CHKRET(time_one(ch, best_var));
   }
-  // candidates within 0.5 % of the fastest are ties at this timing's noise
-  // (a few launches per candidate): among them, keep the one whose schedule
-  // writes the fewest partial bytes (HBM traffic; faster first among equal
-  // bytes)
+  // candidates within 1 % of the fastest count as ties (this timing's noise
+  // is a few tenths of a percent): among them, keep the one whose schedule
+  // writes the fewest partial bytes (HBM traffic, the bandwidth left to other
+  // work; faster first among equal bytes)
   double sel_bytes = 1e300, sel_ms = 1e300;
   for (const Cand& k : cands)
-    if (k.ms <= best_ms * 1.005 && (k.bytes < sel_bytes || (k.bytes == sel_bytes && k.ms < sel_ms))) {
+    if (k.ms <= best_ms * 1.01 && (k.bytes < sel_bytes || (k.bytes == sel_bytes && k.ms < sel_ms))) {
       sel_bytes = k.bytes;
       sel_ms = k.ms;
       best_ch = k.ch;
