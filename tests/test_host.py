@@ -320,6 +320,11 @@ def test_bench_traffic_lookup_labels_its_source():
     assert (10, 1) not in bench.SYM_TRAFFIC_2E20 and (1, 1) in bench.SYM_TRAFFIC_2E20
     assert bench.sym_traffic_2e20(10, 1, rows=4) == (None, None)  # grouped: no fallback to variant 1
     assert all(v not in bench.SYM_SPILLING_VARIANTS for v in (1, 3, 5, 6, 7, 8))
+    # the LDS-source variants (the defaults, and what the autotuner keeps at
+    # 2^20) have their own readings at every block count it times
+    for tile, rows in ((11, 1), (12, 4), (13, 2)):
+        for chunk in (1, 2, 4, 8, 16, 32):
+            assert bench.sym_traffic_2e20(tile, chunk, rows=rows)[1] == "pmc", (tile, chunk)
 
 
 def test_bench_pair_evaluation_count():
