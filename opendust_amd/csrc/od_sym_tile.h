@@ -283,8 +283,9 @@ __device__ __forceinline__ float wave_ror1(float v) {
 // source.  Per step: MT pair terms, 3 DPP moves, one LDS read.
 // PF: read step s + 1's source while step s is evaluated (3 more VGPRs and
 // one address instruction per step).  Measured on MI355X: the all-pairs
-// kernel with 8 target sub-blocks 276.2 -> 269.9 ms at 2^20; the cell
-// kernel with 2 slower, 52.2 -> 57.0 ms at 2^22.
+// kernel with 8 target sub-blocks went from 276.2 to 269.9 ms at 2^20; the
+// cell kernel with 2 target sub-blocks ran slower with it, 52.2 -> 57.0 ms
+// at 2^22, and does without.
 // ls: this lane's entry ls + lane + 64.  (bx, by, bz): added to.
 template <int MT, int CUT, bool PF = false>
 __device__ __forceinline__ void sym_tile_lds(SymT<MT>& T, const float4* ls, float& bx, float& by, float& bz,
