@@ -143,11 +143,21 @@ __global__ __launch_bounds__(128) void probe_sym(float* out, int iters, float a,
   }
   const SymLaw L = VG ? SymLaw{vgpr(a), vgpr(linv), 0.f, 0.f, 0.f} : SymLaw{a, linv, 0.f, 0.f, 0.f};
   const int lane = threadIdx.x & 63;
+  // VG == 2: each wave's sources twice in LDS (sym_tile_lds, read one step ahead)
+  __shared__ float4 lsrc[VG == 2 ? 2 : 1][VG == 2 ? 128 : 1];
   float xj = 0.3f + 1e-3f * lane, yj = -0.2f + 2e-3f * lane, zj = 0.1f - 1e-3f * lane;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for (int i = 0; i < iters; ++i) {
     float bx = 0.f, by = 0.f, bz = 0.f;
-    sym_tile<8, 0, VG>(T, xj, yj, zj, bx, by, bz, L);
+    if constexpr (VG == 2) {
+      float4* ls = lsrc[threadIdx.x >> 6];
+      ls[lane] = ls[lane + 64] = make_float4(xj, yj, zj, 0.f);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      sym_tile_lds<8, 0, true>(T, ls + lane + 64, bx, by, bz, L);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    } else {
+      sym_tile<8, 0, VG>(T, xj, yj, zj, bx, by, bz, L);
+    }
     sx += bx;
     sy += by;
     sz += bz;
@@ -235,18 +245,23 @@ int main() {
   // the symmetric register tile: 2-wave blocks, exactly 2 wps blocks per CU
   // (wps waves per SIMD): each block reserves LDS it does not use, so that no
   // more than 2 wps blocks fit in a CU's LDS, and the grid holds 2 wps blocks
-  // per CU (both forms use at most 80 VGPRs, so up to 6 waves per SIMD fit)
+  // per CU (the forms use at most 80 VGPRs, so up to 6 waves per SIMD fit);
+  // the LDS-source form's own 4 KiB of sources come out of that reserve
   const int sym_iters = 100;
-  for (int vg = 0; vg < 2; ++vg) {
-    printf(" \"sym_tile_%s\": {", vg ? "vgpr" : "dpp_sgpr");
+  const char* sym_names[3] = {"dpp_sgpr", "vgpr", "lds_sources"};
+  for (int vg = 0; vg < 3; ++vg) {
+    printf(" \"sym_tile_%s\": {", sym_names[vg]);
     const int wps_list[3] = {3, 4, 6};
     for (int w = 0; w < 3; ++w) {
       const int per_cu = 2 * wps_list[w];
       const int sym_blocks = p.multiProcessorCount * per_cu;
-      const size_t lds = ((size_t)p.maxSharedMemoryPerMultiProcessor / per_cu) & ~(size_t)1023;
+      const size_t lds = (((size_t)p.maxSharedMemoryPerMultiProcessor / per_cu) & ~(size_t)1023) -
+                         (vg == 2 ? 2 * 128 * sizeof(float4) : 0);
       const double ms = time_ms(
           [&]() {
-            if (vg)
+            if (vg == 2)
+              hipLaunchKernelGGL(probe_sym<2>, dim3(sym_blocks), dim3(128), lds, 0, d, sym_iters, 0.05f, 0.6931f);
+            else if (vg == 1)
               hipLaunchKernelGGL(probe_sym<1>, dim3(sym_blocks), dim3(128), lds, 0, d, sym_iters, 0.05f, 0.6931f);
             else
               hipLaunchKernelGGL(probe_sym<0>, dim3(sym_blocks), dim3(128), lds, 0, d, sym_iters, 0.05f, 0.6931f);
@@ -256,7 +271,7 @@ int main() {
       printf("\"%d_waves_per_simd\": {\"ms\": %.4f, \"unordered_pairs_per_s\": %.5e, \"lds_per_block\": %zu}%s",
              wps_list[w], ms, unordered / (ms * 1e-3), lds, w < 2 ? ", " : "");
     }
-    printf("}%s\n", vg ? "" : ",");
+    printf("}%s\n", vg == 2 ? "" : ",");
   }
   printf("}\n");
   (void)hipFree(d);
