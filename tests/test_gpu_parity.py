@@ -196,10 +196,11 @@ def test_sym_schedule_in_bands(oracle_lib, tile):
 def test_full_size_sym_forces_vs_fp64_gpu():
     """At the benchmark size, 2^20 ions, the FP32c symmetric kernel against the
     plain FP64 row kernel on the GPU (independent code: ordered pairs, FP64
-    law, no partials), for the default variant and schedule (row groups of 4
-    target blocks, 8 source blocks per work-group), for 4 targets per lane
-    with 16 source blocks per work-group, and for one row per work-group
-    (variant 6) with 4.
+    law, no partials), for the default variant and schedule (sources read
+    from LDS, row groups of 4 target blocks, 8 source blocks per work-group),
+    for row groups of 2 with LDS sources (variant 13) and of 4 with register
+    sources (variant 10), for 4 targets per lane with 16 source blocks per
+    work-group, and for one row per work-group (variant 6) with 4.
 
     Two error sources are separated.  (a) Kernel arithmetic: against FP64 on
     the same positions rounded to FP32 in the kernel's units (what the FP32
@@ -227,7 +228,7 @@ def test_full_size_sym_forces_vs_fp64_gpu():
     f64r = e.pair_forces(x32)
     e.close()
     cfg32, _, _ = workloads.prepare(od, alpha=alpha, np_seed=5, precision="fp32c")
-    for tile, ch in ((0, 0), (3, 16), (6, 4), (10, 8)):
+    for tile, ch in ((0, 0), (13, 8), (3, 16), (6, 4), (10, 8)):
         cfg32.sym_tile, cfg32.sym_chunk = tile, ch
         e = eng.Engine(cfg32)
         assert e.pair_info()[0] == MODES["sym"]
